@@ -1,0 +1,159 @@
+// dtm_internal.h — POD parameter blocks passed by value to the gfx950 kernels,
+// and the launcher entry points implemented in dtm_kernels.hip.
+//
+// Device layout (HBM): six field arrays, SoA, C order [nx][ny][nz] with z
+// fastest so a 64-lane wavefront reads 64 consecutive z cells of one (x,y)
+// row.  This replaces the reference's DiamondRag rag structs
+// (params.h:242-262) and their copyrags.cuh shuffles.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dtm {
+
+constexpr int kMaxClasses = 256;
+constexpr int kMaxPoles = 4;
+constexpr int kMaxProbes = 4096;
+
+struct Grid {
+  int nx, ny, nz;
+  long long sx, sy;  // strides of x and y (sz = 1)
+  int per[3];        // periodic axes
+  float rd[3];       // 1/(24 d_a): the (2,4) weights 27/24, -1/24 (defs.h:3-13)
+  float dt;
+};
+
+// CPML: psi lives only in the slabs.  For axis a the two affected
+// components are (a+1)%3 (slot 0) and (a+2)%3 (slot 1).
+struct Cpml {
+  int on;
+  int plo[3], phi[3];
+  const float* bE[3];  // indexed by cell index along the axis (E: odd half-coords)
+  const float* cE[3];
+  const float* bH[3];  // (H: even half-coords)
+  const float* cH[3];
+  float* psiE[3][2];
+  float* psiH[3][2];
+};
+
+// Material class table: class byte per cell -> per-component inverse eps and
+// "special" (dispersive / gyrotropic) flags.  Replaces get_mat_indev and the
+// texture lookups of defs.h:128-182.
+struct ClassEntry {
+  float inv_eps[3];
+  uint8_t mat[3];
+  uint8_t special;  // bit c set: component c uses the ADE path; bit 3: gyro
+};
+
+struct Materials {
+  const uint8_t* cls;  // nullptr => all cells are class 0
+  const ClassEntry* table;
+  int nclass;
+  // continuous-eps design box (inverse design)
+  int d_on, d_i0, d_j0, d_k0, d_ni, d_nj, d_nk;
+  const float* d_inv_eps;
+};
+
+// Lorentz/Drude ADE (dispersion.cpp:41-99) over a bounding box of cells.
+struct AdeMat {
+  int np;
+  float kE[3];
+  float kEJ[kMaxPoles][2];
+  float kJ[kMaxPoles][5];
+};
+struct GyroMat {
+  float A[9], B[9];
+};
+struct Ade {
+  int on;
+  int i0, j0, k0, ni, nj, nk;  // bounding box
+  long long bsx, bsy;
+  float* Em[3];
+  float* Jp[3][kMaxPoles];
+  float* Jm[3][kMaxPoles];
+  const AdeMat* mats;  // indexed by material id
+  int gyro_on;
+  float* Jg[3];
+  const GyroMat* gmats;
+};
+
+// TF/SF: integer inSF bounds on half-coordinates per axis (computed on the
+// host with the same double predicate as signal.hpp:153-160) and the
+// incident-field parameters (double).
+struct Tfsf {
+  int active;
+  int kind, waveform, sf_inside;
+  int hlo[3], hhi[3];  // inside <=> hlo <= h <= hhi on every axis
+  // shell test: cell in [clo,chi] on every axis and, on some axis a, in one
+  // of the face bands [fl0,fl1] or [fh0,fh1]
+  int clo[3], chi[3], fl0[3], fl1[3], fh0[3], fh1[3];
+  double half_d[3];    // d_a / 2
+  double w, kEnv, cSrc, Imp, t0, tw, amp;
+  double st, ct, sph, cph, sps, cps;  // plane wave angles
+  double kx, ky, kz, pos[3];
+  double dip[3];  // dipole position
+};
+
+template <int STORE> struct Fields;
+template <> struct Fields<0> {
+  float* E[3];
+  float* H[3];
+};
+template <> struct Fields<1> {
+  __half* E[3];
+  __half* H[3];
+};
+
+struct Scales {  // FP16 storage: value = stored * inv, stored = value * scale
+  float se[3], sh[3], ie[3], ih[3];
+};
+
+struct Region {  // a box of cells [i0,i1) x [j0,j1) x [k0,k1)
+  int i0, i1, j0, j1, k0, k1;
+};
+
+struct ProbeDesc {
+  int kind, comp, i, j, k;
+};
+
+struct Probes {
+  int n;
+  const ProbeDesc* desc;
+  float* series;  // [row][n]
+  int row;
+  int nfreq;
+  const double* omega;
+  double* dft;  // [n][nfreq][2]
+  double dt;
+  long long step;
+};
+
+struct Soft {
+  int n;
+  int comp[16];
+  long long q[16];
+  float val[16];  // dt * inv_eps * J(t)
+};
+
+// Launchers (dtm_kernels.hip).  All enqueue on `st`.
+// Which region groups a launch covers: bit 0 interior, bit 1 edge slabs.
+void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                     const Tfsf& tf, double t_inc, const Region* regs, int nreg, const Region& main,
+                     hipStream_t st);
+void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                     const Materials& mt, const Ade& ade, const Tfsf& tf, double t_inc,
+                     const Region* regs, int nreg, const Region& main, hipStream_t st);
+void launch_soft(int store, const Grid& g, const void* fields, const Scales& sc, const Soft& so,
+                 hipStream_t st);
+void launch_record(int store, const Grid& g, const void* fields, const Scales& sc, const Probes& pr,
+                   hipStream_t st);
+void launch_energy(int store, const Grid& g, const void* fields, const Scales& sc,
+                   const Materials& mt, double* out, hipStream_t st);
+void launch_to_float(int store, const void* src, float scale_inv, float* dst, long long n,
+                     hipStream_t st);
+void launch_from_float(int store, const float* src, float scale, void* dst, long long n,
+                       hipStream_t st);
+const char* kernel_variant_name(int store);
+
+}  // namespace dtm

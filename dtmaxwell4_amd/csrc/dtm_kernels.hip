@@ -1,0 +1,677 @@
+// dtm_kernels.hip — hand-written gfx950 kernels of the FDTD stepping path.
+//
+// One launch per half-step ("two-pass" sweep).  Each launch covers a set of
+// boxes (regions): the interior box where every x/y stencil neighbour exists
+// and only z can touch a boundary or a CPML slab, and up to four x/y edge
+// slabs handled with full boundary logic — the zone split of the reference's
+// Window::calcDtorres (window.hpp:126-145) without its host loop.
+//
+// Thread mapping: 256 threads = 64 (z, one wavefront) x 4 (y).  Blocks are
+// remapped so that each of the 8 XCDs sweeps a contiguous run of (x,y) rows,
+// keeping the x/y stencil neighbours of a block in that XCD's L2.
+//
+// Update equations (genAlg.py:182-301, constants defs.h:3-13), on the cell
+// lattice of include/dtmaxwell4.h:
+//   H^{n+1/2} = H^{n-1/2} - dt curl E^n          (body diamond, time 0)
+//   E^{n+1}   = E^n + dt/eps curl H^{n+1/2}      (nose diamond, time 1/2)
+// with the (2,4) difference D f = [27 (f_{+1/2} - f_{-1/2}) - (f_{+3/2} - f_{-3/2})]/(24 d).
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+
+#include "dtm_internal.h"
+
+namespace dtm {
+
+constexpr int BZ = 64;  // z extent of a block (one wavefront)
+constexpr int BY = 4;   // y extent of a block
+constexpr int NXCD = 8;
+
+struct RegionSet {
+  int n;
+  Region r[6];
+  int nbk[6], nbj[6];
+  int start[7];
+  int total;  // logical blocks
+  int per;    // logical blocks per XCD
+};
+
+static RegionSet make_set(const Region* regs, int nreg) {
+  RegionSet s{};
+  int acc = 0;
+  int m = 0;
+  for (int t = 0; t < nreg && m < 6; t++) {
+    const Region& r = regs[t];
+    if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
+    s.r[m] = r;
+    s.nbk[m] = (r.k1 - r.k0 + BZ - 1) / BZ;
+    s.nbj[m] = (r.j1 - r.j0 + BY - 1) / BY;
+    s.start[m] = acc;
+    acc += s.nbk[m] * s.nbj[m] * (r.i1 - r.i0);
+    m++;
+  }
+  s.n = m;
+  s.start[m] = acc;
+  s.total = acc;
+  s.per = (acc + NXCD - 1) / NXCD;
+  return s;
+}
+
+// Maps the hardware block id to (region, cell).  Returns false for padding
+// blocks and for threads outside their region.
+__device__ __forceinline__ bool locate(const RegionSet& rs, int& i, int& j, int& k) {
+  const int b = blockIdx.x;
+  const int L = (b % NXCD) * rs.per + b / NXCD;  // XCD-contiguous logical id
+  if (L >= rs.total) return false;
+  int r = 0;
+  while (r + 1 < rs.n && L >= rs.start[r + 1]) r++;
+  int loc = L - rs.start[r];
+  const int bk = loc % rs.nbk[r];
+  loc /= rs.nbk[r];
+  const int bj = loc % rs.nbj[r];
+  const int bi = loc / rs.nbj[r];
+  const Region& R = rs.r[r];
+  k = R.k0 + bk * BZ + (int)threadIdx.x;
+  j = R.j0 + bj * BY + (int)threadIdx.y;
+  i = R.i0 + bi;
+  return k < R.k1 && j < R.j1 && i < R.i1;
+}
+
+// ------------------------------------------------------- storage helpers
+__device__ __forceinline__ float ld(const float* F, long long q, float) { return F[q]; }
+__device__ __forceinline__ float ld(const __half* F, long long q, float inv) {
+  return __half2float(F[q]) * inv;
+}
+__device__ __forceinline__ void st(float* F, long long q, float v, float) { F[q] = v; }
+__device__ __forceinline__ void st(__half* F, long long q, float v, float s) {
+  F[q] = __float2half_rn(v * s);
+}
+template <typename T>
+__device__ __forceinline__ float ldn(const T* F, long long q, bool ok, float inv) {
+  return ok ? ld(F, q, inv) : 0.0f;
+}
+
+// Neighbour index along one axis with the boundary rule: -1 => zero field.
+__device__ __forceinline__ int nbr(int v, int o, int n, int per) {
+  int w = v + o;
+  if (w >= 0 && w < n) return w;
+  if (!per) return -1;
+  return w < 0 ? w + n : w - n;
+}
+
+// (2,4) difference with weights (+1, -27, +27, -1) on the samples at
+// half-offsets (-3, -1, +1, +3) (genAlg.py:196-198, sign folded).
+__device__ __forceinline__ float d24(float f0, float f1, float f2, float f3) {
+  return 27.0f * (f2 - f1) - (f3 - f0);
+}
+
+// ------------------------------------------------------- incident fields
+__device__ void wave3(int waveform, double p, double w, double kEnv, double t0, double tw,
+                      double& f0, double& f1, double& f2) {
+  if (waveform == 0) {  // raised-cosine window (signal.hpp:71-77)
+    const double kp = p * kEnv;
+    if (kp < 0.0 || kp > 2.0 * M_PI) {
+      f0 = f1 = f2 = 0.0;
+      return;
+    }
+    double sp, cp, sk, ck;
+    sincos(p, &sp, &cp);
+    sincos(kp, &sk, &ck);
+    f0 = sp * 0.5 * (1.0 - ck);
+    f1 = 0.5 * cp * (1.0 - ck) + 0.5 * kEnv * sp * sk;
+    f2 = -0.5 * sp * (1.0 - ck) + kEnv * cp * sk + 0.5 * kEnv * kEnv * sp * ck;
+  } else {  // Gaussian-modulated sine
+    const double a = 1.0 / (w * tw);
+    const double u = (p / w - t0) / tw;
+    const double g = exp(-u * u);
+    const double g1 = -2.0 * u * a * g;
+    const double g2 = (4.0 * u * u * a * a - 2.0 * a * a) * g;
+    double sp, cp;
+    sincos(p, &sp, &cp);
+    f0 = g * sp;
+    f1 = g1 * sp + g * cp;
+    f2 = g2 * sp + 2.0 * g1 * cp - g * sp;
+  }
+}
+
+// Incident value of component comp (0..5) at half-coordinates h, time t.
+// Dipole: signal.hpp:58-96; plane wave: signal.hpp:17-49.
+__device__ __noinline__ double incident(const Tfsf& s, int comp, int hx, int hy, int hz, double t) {
+  const double x = hx * s.half_d[0], y = hy * s.half_d[1], z = hz * s.half_d[2];
+  if (s.kind == 1) {
+    double n0 = x - s.dip[0], n1 = y - s.dip[1], n2 = z - s.dip[2];
+    const double R = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    const double c = s.cSrc, w = s.w;
+    double f0, f1, f2;
+    wave3(s.waveform, w * t - w / c * R, w, s.kEnv, s.t0, s.tw, f0, f1, f2);
+    const double d = s.amp * f0, ds = s.amp * w * f1, dss = s.amp * w * w * f2;
+    n0 /= R;
+    n1 /= R;
+    n2 /= R;
+    // vectors along z: d = (0,0,d)
+    if (comp < 3) {
+      // (dss x n) x n for dss = (0,0,dss): = dss*(n2*n - e_z)
+      const double nc = comp == 0 ? n0 : (comp == 1 ? n1 : n2);
+      const double ez = comp == 2 ? 1.0 : 0.0;
+      const double t2 = dss * (n2 * nc - ez);
+      return c * (t2 / (c * c * R) + (3.0 * ds * n2 * nc - ds * ez) / (c * R * R) +
+                  (3.0 * d * n2 * nc - d * ez) / (R * R * R));
+    } else {
+      // (v e_z) x n = v (-n1, n0, 0)
+      const int a = comp - 3;
+      const double cz = a == 0 ? -n1 : (a == 1 ? n0 : 0.0);
+      return dss * cz / (c * c * R) + ds * cz / (c * R * R);
+    }
+  } else {
+    const double phase = s.w * t - s.kx * (x - s.pos[0]) - s.ky * (y - s.pos[1]) - s.kz * (z - s.pos[2]);
+    double f0, f1, f2;
+    wave3(s.waveform, phase, s.w, s.kEnv, s.t0, s.tw, f0, f1, f2);
+    const double sv = s.amp * f0;
+    const double Ephi = s.Imp * sv * s.cps, Eth = s.Imp * sv * s.sps;
+    const double Hth = -sv * s.cps, Hphi = sv * s.sps;
+    switch (comp) {
+      case 0: return -Ephi * s.sph + Eth * s.ct * s.cph;
+      case 1: return Ephi * s.cph + Eth * s.ct * s.sph;
+      case 2: return -Eth * s.st;
+      case 3: return -Hphi * s.sph + Hth * s.ct * s.cph;
+      case 4: return Hphi * s.cph + Hth * s.ct * s.sph;
+      default: return -Hth * s.st;
+    }
+  }
+}
+
+__device__ __forceinline__ bool in_sf(const Tfsf& s, int hx, int hy, int hz) {
+  const bool inside = hx >= s.hlo[0] && hx <= s.hhi[0] && hy >= s.hlo[1] && hy <= s.hhi[1] &&
+                      hz >= s.hlo[2] && hz <= s.hhi[2];
+  return s.sf_inside ? inside : !inside;
+}
+
+// Component living at half-coordinates (genAlg.py:24 typing).
+__device__ __forceinline__ int comp_at(int hx, int hy, int hz) {
+  const int o0 = hx & 1, o1 = hy & 1, o2 = hz & 1;
+  if (o0 + o1 + o2 == 2) return o0 == 0 ? 0 : (o1 == 0 ? 1 : 2);
+  return o0 ? 3 : (o1 ? 4 : 5);
+}
+
+// Sum over the 4 stencil neighbours along axis a of component-at-(hp) of the
+// TF/SF corrections (genAlg.py:219-221) weighted by (+1,-27,+27,-1).
+__device__ __noinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t) {
+  const bool psf = in_sf(s, hx, hy, hz);
+  const int ho[4] = {-3, -1, 1, 3};
+  const double wt[4] = {1.0, -27.0, 27.0, -1.0};
+  double acc = 0.0;
+  for (int q = 0; q < 4; q++) {
+    int h0 = hx, h1 = hy, h2 = hz;
+    if (a == 0) h0 += ho[q];
+    else if (a == 1) h1 += ho[q];
+    else h2 += ho[q];
+    const bool qsf = in_sf(s, h0, h1, h2);
+    if (qsf == psf) continue;
+    const double v = incident(s, comp_at(h0, h1, h2), h0, h1, h2, t);
+    acc += wt[q] * ((qsf && !psf) ? v : -v);
+  }
+  return (float)acc;
+}
+
+__device__ __forceinline__ bool near_face(const Tfsf& s, int a, int v) {
+  return (v >= s.fl0[a] && v <= s.fl1[a]) || (v >= s.fh0[a] && v <= s.fh1[a]);
+}
+__device__ __forceinline__ bool in_shell(const Tfsf& s, int i, int j, int k) {
+  if (i < s.clo[0] || i > s.chi[0] || j < s.clo[1] || j > s.chi[1] || k < s.clo[2] || k > s.chi[2]) return false;
+  return near_face(s, 0, i) || near_face(s, 1, j) || near_face(s, 2, k);
+}
+
+// -------------------------------------------------------- CPML helpers
+__device__ __forceinline__ int slab_index(int v, int n, int plo, int phi) {
+  if (v < plo) return v;
+  if (v >= n - phi) return plo + v - (n - phi);
+  return -1;
+}
+
+__device__ __forceinline__ long long psi_index(const Grid& g, const Cpml& p, int a, int s, int i, int j,
+                                               int k) {
+  const int P = p.plo[a] + p.phi[a];
+  if (a == 0) return ((long long)s * g.ny + j) * g.nz + k;
+  if (a == 1) return ((long long)i * P + s) * g.nz + k;
+  return ((long long)i * g.ny + j) * P + s;
+}
+
+// psi <- b psi + c D ; D <- D + psi   (CPML, kappa=1, alpha=0)
+__device__ __forceinline__ void cpml_apply(float* psi, long long pq, float b, float c, float& D) {
+  float v = b * psi[pq] + c * D;
+  psi[pq] = v;
+  D += v;
+}
+
+// ================================================================ H sweep
+template <int STORE, bool GEN>
+__global__ __launch_bounds__(256) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Tfsf tf,
+                                                  double tinc, RegionSet rs) {
+  int i, j, k;
+  if (!locate(rs, i, j, k)) return;
+  const long long sx = g.sx, sy = g.sy;
+  const long long q = i * sx + j * sy + k;
+  // z neighbours (offsets -2, -1, +1)
+  const int km2 = nbr(k, -2, g.nz, g.per[2]), km1 = nbr(k, -1, g.nz, g.per[2]),
+            kp1 = nbr(k, 1, g.nz, g.per[2]);
+  const long long rowq = q - k;
+  long long xm2, xm1, xp1, ym2, ym1, yp1;
+  bool oxm2 = true, oxm1 = true, oxp1 = true, oym2 = true, oym1 = true, oyp1 = true;
+  if (GEN) {
+    const int a = nbr(i, -2, g.nx, g.per[0]), b = nbr(i, -1, g.nx, g.per[0]), c = nbr(i, 1, g.nx, g.per[0]);
+    oxm2 = a >= 0; oxm1 = b >= 0; oxp1 = c >= 0;
+    xm2 = q + (long long)(a - i) * sx; xm1 = q + (long long)(b - i) * sx; xp1 = q + (long long)(c - i) * sx;
+    const int d = nbr(j, -2, g.ny, g.per[1]), e = nbr(j, -1, g.ny, g.per[1]), h = nbr(j, 1, g.ny, g.per[1]);
+    oym2 = d >= 0; oym1 = e >= 0; oyp1 = h >= 0;
+    ym2 = q + (long long)(d - j) * sy; ym1 = q + (long long)(e - j) * sy; yp1 = q + (long long)(h - j) * sy;
+  } else {
+    xm2 = q - 2 * sx; xm1 = q - sx; xp1 = q + sx;
+    ym2 = q - 2 * sy; ym1 = q - sy; yp1 = q + sy;
+  }
+  const auto* Ex = f.E[0];
+  const auto* Ey = f.E[1];
+  const auto* Ez = f.E[2];
+  const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
+  const float ez0 = ld(Ez, q, iez), ey0 = ld(Ey, q, iey), ex0 = ld(Ex, q, iex);
+  // derivatives of E (backward stencil: samples at -2,-1,0,+1)
+  float dyEz = g.rd[1] * d24(ldn(Ez, ym2, oym2, iez), ldn(Ez, ym1, oym1, iez), ez0, ldn(Ez, yp1, oyp1, iez));
+  float dxEz = g.rd[0] * d24(ldn(Ez, xm2, oxm2, iez), ldn(Ez, xm1, oxm1, iez), ez0, ldn(Ez, xp1, oxp1, iez));
+  float dzEy = g.rd[2] * d24(ldn(Ey, rowq + km2, km2 >= 0, iey), ldn(Ey, rowq + km1, km1 >= 0, iey), ey0,
+                             ldn(Ey, rowq + kp1, kp1 >= 0, iey));
+  float dxEy = g.rd[0] * d24(ldn(Ey, xm2, oxm2, iey), ldn(Ey, xm1, oxm1, iey), ey0, ldn(Ey, xp1, oxp1, iey));
+  float dzEx = g.rd[2] * d24(ldn(Ex, rowq + km2, km2 >= 0, iex), ldn(Ex, rowq + km1, km1 >= 0, iex), ex0,
+                             ldn(Ex, rowq + kp1, kp1 >= 0, iex));
+  float dyEx = g.rd[1] * d24(ldn(Ex, ym2, oym2, iex), ldn(Ex, ym1, oym1, iex), ex0, ldn(Ex, yp1, oyp1, iex));
+
+  if (tf.active && in_shell(tf, i, j, k)) {
+    // Hx (2i+1,2j,2k), Hy (2i,2j+1,2k), Hz (2i,2j,2k+1)
+    dyEz += g.rd[1] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k, 1, tinc);
+    dzEy += g.rd[2] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k, 2, tinc);
+    dzEx += g.rd[2] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k, 2, tinc);
+    dxEz += g.rd[0] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k, 0, tinc);
+    dxEy += g.rd[0] * tfsf_dcorr(tf, 2 * i, 2 * j, 2 * k + 1, 0, tinc);
+    dyEx += g.rd[1] * tfsf_dcorr(tf, 2 * i, 2 * j, 2 * k + 1, 1, tinc);
+  }
+  if (pm.on) {
+    // z axis (all regions): slot 0 = comp x (Hx: D_z Ey), slot 1 = comp y (Hy: D_z Ex)
+    const int sz = slab_index(k, g.nz, pm.plo[2], pm.phi[2]);
+    if (sz >= 0) {
+      const float b = pm.bH[2][k], c = pm.cH[2][k];
+      const long long pq = psi_index(g, pm, 2, sz, i, j, k);
+      cpml_apply(pm.psiH[2][0], pq, b, c, dzEy);
+      cpml_apply(pm.psiH[2][1], pq, b, c, dzEx);
+    }
+    if (GEN) {
+      const int sxi = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
+      if (sxi >= 0) {  // x axis: slot 0 = comp y (Hy: D_x Ez), slot 1 = comp z (Hz: D_x Ey)
+        const float b = pm.bH[0][i], c = pm.cH[0][i];
+        const long long pq = psi_index(g, pm, 0, sxi, i, j, k);
+        cpml_apply(pm.psiH[0][0], pq, b, c, dxEz);
+        cpml_apply(pm.psiH[0][1], pq, b, c, dxEy);
+      }
+      const int syi = slab_index(j, g.ny, pm.plo[1], pm.phi[1]);
+      if (syi >= 0) {  // y axis: slot 0 = comp z (Hz: D_y Ex), slot 1 = comp x (Hx: D_y Ez)
+        const float b = pm.bH[1][j], c = pm.cH[1][j];
+        const long long pq = psi_index(g, pm, 1, syi, i, j, k);
+        cpml_apply(pm.psiH[1][0], pq, b, c, dyEx);
+        cpml_apply(pm.psiH[1][1], pq, b, c, dyEz);
+      }
+    }
+  }
+  auto* Hx = f.H[0];
+  auto* Hy = f.H[1];
+  auto* Hz = f.H[2];
+  const float dt = g.dt;
+  st(Hx, q, ld(Hx, q, sc.ih[0]) - dt * (dyEz - dzEy), sc.sh[0]);
+  st(Hy, q, ld(Hy, q, sc.ih[1]) - dt * (dzEx - dxEz), sc.sh[1]);
+  st(Hz, q, ld(Hz, q, sc.ih[2]) - dt * (dxEy - dyEx), sc.sh[2]);
+}
+
+// ======================================================== E sweep helpers
+// Dispersive / gyrotropic cells: countDisp (dispersion.cpp:85-99) and the
+// magnetised-plasma current.  curl[] is curl H (with CPML / TF/SF applied).
+template <typename T>
+__device__ __noinline__ void e_special(const Grid& g, T* E0, T* E1, T* E2, const Scales& sc, const Ade& ade,
+                                       const ClassEntry& ce, const float inv_eps_override[3], bool over,
+                                       long long q, int i, int j, int k, float c0, float c1, float c2) {
+  T* E[3] = {E0, E1, E2};
+  float curl[3] = {c0, c1, c2};
+  const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
+  float En[3];
+  for (int c = 0; c < 3; c++) En[c] = ld(E[c], q, sc.ie[c]);
+  if ((ce.special & 8) && ade.gyro_on) {
+    const GyroMat& gm = ade.gmats[ce.mat[0]];
+    float J0[3] = {ade.Jg[0][b], ade.Jg[1][b], ade.Jg[2][b]};
+    for (int r = 0; r < 3; r++) {
+      float v = 0.f;
+      for (int t = 0; t < 3; t++) v += gm.A[3 * r + t] * J0[t] + gm.B[3 * r + t] * En[t];
+      ade.Jg[r][b] = v;
+      curl[r] -= v;
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    if (!over && (ce.special & (1 << c))) {
+      const AdeMat& am = ade.mats[ce.mat[c]];
+      const float Ep = En[c];
+      const float Em = ade.Em[c][b];
+      const float rotdt = g.dt * curl[c];
+      float sum = 0.f;
+      for (int p = 0; p < am.np; p++) sum += am.kEJ[p][0] * ade.Jp[c][p][b] + am.kEJ[p][1] * ade.Jm[c][p][b];
+      const float Enew = am.kE[0] * Em + am.kE[1] * Ep + am.kE[2] * rotdt - sum;
+      for (int p = 0; p < am.np; p++) {
+        const float jp = ade.Jp[c][p][b], jm = ade.Jm[c][p][b];
+        const float jn = am.kJ[p][0] * jp + am.kJ[p][1] * jm + am.kJ[p][2] * Enew + am.kJ[p][3] * Em +
+                         am.kJ[p][4] * Ep;
+        ade.Jm[c][p][b] = jp;
+        ade.Jp[c][p][b] = jn;
+      }
+      ade.Em[c][b] = Ep;
+      st(E[c], q, Enew, sc.se[c]);
+    } else {
+      const float ie = over ? inv_eps_override[c] : ce.inv_eps[c];
+      st(E[c], q, En[c] + g.dt * ie * curl[c], sc.se[c]);
+    }
+  }
+}
+
+// ================================================================ E sweep
+template <int STORE, bool GEN>
+__global__ __launch_bounds__(256) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
+                                                  Ade ade, Tfsf tf, double tinc, RegionSet rs) {
+  __shared__ ClassEntry tbl[kMaxClasses];
+  const int tid = threadIdx.x + threadIdx.y * BZ;
+  for (int t = tid; t < mt.nclass; t += BZ * BY) tbl[t] = mt.table[t];
+  __syncthreads();
+  int i, j, k;
+  if (!locate(rs, i, j, k)) return;
+  const long long sx = g.sx, sy = g.sy;
+  const long long q = i * sx + j * sy + k;
+  // z neighbours (offsets -1, +1, +2)
+  const int km1 = nbr(k, -1, g.nz, g.per[2]), kp1 = nbr(k, 1, g.nz, g.per[2]), kp2 = nbr(k, 2, g.nz, g.per[2]);
+  const long long rowq = q - k;
+  long long xm1, xp1, xp2, ym1, yp1, yp2;
+  bool oxm1 = true, oxp1 = true, oxp2 = true, oym1 = true, oyp1 = true, oyp2 = true;
+  if (GEN) {
+    const int a = nbr(i, -1, g.nx, g.per[0]), b = nbr(i, 1, g.nx, g.per[0]), c = nbr(i, 2, g.nx, g.per[0]);
+    oxm1 = a >= 0; oxp1 = b >= 0; oxp2 = c >= 0;
+    xm1 = q + (long long)(a - i) * sx; xp1 = q + (long long)(b - i) * sx; xp2 = q + (long long)(c - i) * sx;
+    const int d = nbr(j, -1, g.ny, g.per[1]), e = nbr(j, 1, g.ny, g.per[1]), h = nbr(j, 2, g.ny, g.per[1]);
+    oym1 = d >= 0; oyp1 = e >= 0; oyp2 = h >= 0;
+    ym1 = q + (long long)(d - j) * sy; yp1 = q + (long long)(e - j) * sy; yp2 = q + (long long)(h - j) * sy;
+  } else {
+    xm1 = q - sx; xp1 = q + sx; xp2 = q + 2 * sx;
+    ym1 = q - sy; yp1 = q + sy; yp2 = q + 2 * sy;
+  }
+  const auto* Hx = f.H[0];
+  const auto* Hy = f.H[1];
+  const auto* Hz = f.H[2];
+  const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
+  const float hz0 = ld(Hz, q, ihz), hy0 = ld(Hy, q, ihy), hx0 = ld(Hx, q, ihx);
+  // derivatives of H (forward stencil: samples at -1,0,+1,+2)
+  float dyHz = g.rd[1] * d24(ldn(Hz, ym1, oym1, ihz), hz0, ldn(Hz, yp1, oyp1, ihz), ldn(Hz, yp2, oyp2, ihz));
+  float dxHz = g.rd[0] * d24(ldn(Hz, xm1, oxm1, ihz), hz0, ldn(Hz, xp1, oxp1, ihz), ldn(Hz, xp2, oxp2, ihz));
+  float dzHy = g.rd[2] * d24(ldn(Hy, rowq + km1, km1 >= 0, ihy), hy0, ldn(Hy, rowq + kp1, kp1 >= 0, ihy),
+                             ldn(Hy, rowq + kp2, kp2 >= 0, ihy));
+  float dxHy = g.rd[0] * d24(ldn(Hy, xm1, oxm1, ihy), hy0, ldn(Hy, xp1, oxp1, ihy), ldn(Hy, xp2, oxp2, ihy));
+  float dzHx = g.rd[2] * d24(ldn(Hx, rowq + km1, km1 >= 0, ihx), hx0, ldn(Hx, rowq + kp1, kp1 >= 0, ihx),
+                             ldn(Hx, rowq + kp2, kp2 >= 0, ihx));
+  float dyHx = g.rd[1] * d24(ldn(Hx, ym1, oym1, ihx), hx0, ldn(Hx, yp1, oyp1, ihx), ldn(Hx, yp2, oyp2, ihx));
+
+  if (tf.active && in_shell(tf, i, j, k)) {
+    // Ex (2i,2j+1,2k+1), Ey (2i+1,2j,2k+1), Ez (2i+1,2j+1,2k)
+    dyHz += g.rd[1] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k + 1, 1, tinc);
+    dzHy += g.rd[2] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k + 1, 2, tinc);
+    dzHx += g.rd[2] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k + 1, 2, tinc);
+    dxHz += g.rd[0] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k + 1, 0, tinc);
+    dxHy += g.rd[0] * tfsf_dcorr(tf, 2 * i + 1, 2 * j + 1, 2 * k, 0, tinc);
+    dyHx += g.rd[1] * tfsf_dcorr(tf, 2 * i + 1, 2 * j + 1, 2 * k, 1, tinc);
+  }
+  if (pm.on) {
+    const int sz = slab_index(k, g.nz, pm.plo[2], pm.phi[2]);
+    if (sz >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
+      const float b = pm.bE[2][k], c = pm.cE[2][k];
+      const long long pq = psi_index(g, pm, 2, sz, i, j, k);
+      cpml_apply(pm.psiE[2][0], pq, b, c, dzHy);
+      cpml_apply(pm.psiE[2][1], pq, b, c, dzHx);
+    }
+    if (GEN) {
+      const int sxi = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
+      if (sxi >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
+        const float b = pm.bE[0][i], c = pm.cE[0][i];
+        const long long pq = psi_index(g, pm, 0, sxi, i, j, k);
+        cpml_apply(pm.psiE[0][0], pq, b, c, dxHz);
+        cpml_apply(pm.psiE[0][1], pq, b, c, dxHy);
+      }
+      const int syi = slab_index(j, g.ny, pm.plo[1], pm.phi[1]);
+      if (syi >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
+        const float b = pm.bE[1][j], c = pm.cE[1][j];
+        const long long pq = psi_index(g, pm, 1, syi, i, j, k);
+        cpml_apply(pm.psiE[1][0], pq, b, c, dyHx);
+        cpml_apply(pm.psiE[1][1], pq, b, c, dyHz);
+      }
+    }
+  }
+  const float cx = dyHz - dzHy, cy = dzHx - dxHz, cz = dxHy - dyHx;
+  const ClassEntry& ce = tbl[mt.cls ? mt.cls[q] : 0];
+  float ie[3] = {ce.inv_eps[0], ce.inv_eps[1], ce.inv_eps[2]};
+  bool over = false;
+  if (mt.d_on) {
+    const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
+    if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) {
+      const float v = mt.d_inv_eps[((long long)di * mt.d_nj + dj) * mt.d_nk + dk];
+      ie[0] = ie[1] = ie[2] = v;
+      over = true;
+    }
+  }
+  auto* Ex = f.E[0];
+  auto* Ey = f.E[1];
+  auto* Ez = f.E[2];
+  if (ce.special) {
+    e_special(g, Ex, Ey, Ez, sc, ade, ce, ie, over, q, i, j, k, cx, cy, cz);
+    return;
+  }
+  const float dt = g.dt;
+  st(Ex, q, ld(Ex, q, sc.ie[0]) + dt * ie[0] * cx, sc.se[0]);
+  st(Ey, q, ld(Ey, q, sc.ie[1]) + dt * ie[1] * cy, sc.se[1]);
+  st(Ez, q, ld(Ez, q, sc.ie[2]) + dt * ie[2] * cz, sc.se[2]);
+}
+
+// ======================================================== small kernels
+template <int STORE>
+__global__ void k_soft(Fields<STORE> f, Scales sc, Soft so) {
+  const int t = threadIdx.x;
+  if (t >= so.n) return;
+  const int c = so.comp[t];
+  const long long q = so.q[t];
+  st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) - so.val[t], sc.se[c]);
+}
+
+template <int STORE>
+__device__ float fval(const Grid& g, const Fields<STORE>& f, const Scales& sc, int comp, int i, int j, int k) {
+  const int ii = nbr(i, 0, g.nx, g.per[0]), jj = nbr(j, 0, g.ny, g.per[1]), kk = nbr(k, 0, g.nz, g.per[2]);
+  if (ii < 0 || jj < 0 || kk < 0) return 0.f;
+  const long long q = ii * g.sx + jj * g.sy + kk;
+  return comp < 3 ? ld(f.E[comp], q, sc.ie[comp]) : ld(f.H[comp - 3], q, sc.ih[comp - 3]);
+}
+
+// Reference sensor interpolation (sensor.h:64-71) on the cell lattice.
+template <int STORE>
+__device__ float sensor_val(const Grid& g, const Fields<STORE>& f, const Scales& sc, int comp, int x, int y,
+                            int z) {
+  const int zM = (z == 0) ? z + 1 : z - 1;
+  switch (comp) {
+    case 0:
+      return 0.25f * (fval(g, f, sc, 0, x, y - 1, zM) + fval(g, f, sc, 0, x, y - 1, z) +
+                      fval(g, f, sc, 0, x, y, zM) + fval(g, f, sc, 0, x, y, z));
+    case 1:
+      return 0.25f * (fval(g, f, sc, 1, x - 1, y, zM) + fval(g, f, sc, 1, x - 1, y, z) +
+                      fval(g, f, sc, 1, x, y, zM) + fval(g, f, sc, 1, x, y, z));
+    case 2:
+      return 0.25f * (fval(g, f, sc, 2, x - 1, y - 1, z) + fval(g, f, sc, 2, x - 1, y, z) +
+                      fval(g, f, sc, 2, x, y - 1, z) + fval(g, f, sc, 2, x, y, z));
+    case 3: return 0.5f * (fval(g, f, sc, 3, x - 1, y, z) + fval(g, f, sc, 3, x, y, z));
+    case 4: return 0.5f * (fval(g, f, sc, 4, x, y - 1, z) + fval(g, f, sc, 4, x, y, z));
+    default: return 0.5f * (fval(g, f, sc, 5, x, y, zM) + fval(g, f, sc, 5, x, y, z));
+  }
+}
+
+template <int STORE>
+__global__ void k_record(Grid g, Fields<STORE> f, Scales sc, Probes pr) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pr.n) return;
+  const ProbeDesc d = pr.desc[p];
+  const float v = d.kind == 0 ? fval(g, f, sc, d.comp, d.i, d.j, d.k) : sensor_val(g, f, sc, d.comp, d.i, d.j, d.k);
+  pr.series[(long long)pr.row * pr.n + p] = v;
+  if (pr.nfreq > 0) {
+    const double t = d.comp < 3 ? (double)(pr.step + 1) * pr.dt : ((double)pr.step + 0.5) * pr.dt;
+    double* acc = pr.dft + (long long)p * pr.nfreq * 2;
+    for (int fq = 0; fq < pr.nfreq; fq++) {
+      double s, c;
+      sincos(pr.omega[fq] * t, &s, &c);
+      acc[2 * fq] += (double)v * c * pr.dt;
+      acc[2 * fq + 1] += (double)v * s * pr.dt;
+    }
+  }
+}
+
+template <int STORE>
+__global__ void k_energy(Grid g, Fields<STORE> f, Scales sc, Materials mt, double* out) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  double acc = 0.0;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+    const ClassEntry& ce = mt.table[mt.cls ? mt.cls[q] : 0];
+    for (int c = 0; c < 3; c++) {
+      const float e = ld(f.E[c], q, sc.ie[c]);
+      const float h = ld(f.H[c], q, sc.ih[c]);
+      acc += (double)e * e / ce.inv_eps[c] + (double)h * h;
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+__global__ void k_to_float_h(const __half* src, float inv, float* dst, long long n) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    dst[q] = __half2float(src[q]) * inv;
+}
+__global__ void k_from_float_h(const float* src, float s, __half* dst, long long n) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    dst[q] = __float2half_rn(src[q] * s);
+}
+
+// ============================================================ launchers
+static void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fprintf(stderr, "dtm: launch of %s failed: %s\n", what, hipGetErrorString(e));
+}
+
+template <int STORE>
+static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Tfsf& tf, double tinc,
+                  const Region* regs, int nreg, const Region& main, hipStream_t st) {
+  const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
+  const dim3 blk(BZ, BY, 1);
+  RegionSet rm = make_set(&main, main.i1 > main.i0 ? 1 : 0);
+  if (rm.total > 0) {
+    hipLaunchKernelGGL((k_update_h<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, tf, tinc, rm);
+    check_launch("k_update_h<main>");
+  }
+  RegionSet rg = make_set(regs, nreg);
+  if (rg.total > 0) {
+    hipLaunchKernelGGL((k_update_h<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, tf, tinc, rg);
+    check_launch("k_update_h<edge>");
+  }
+}
+
+template <int STORE>
+static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
+                  const Ade& ade, const Tfsf& tf, double tinc, const Region* regs, int nreg, const Region& main,
+                  hipStream_t st) {
+  const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
+  const dim3 blk(BZ, BY, 1);
+  RegionSet rm = make_set(&main, main.i1 > main.i0 ? 1 : 0);
+  if (rm.total > 0) {
+    hipLaunchKernelGGL((k_update_e<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, tf, tinc,
+                       rm);
+    check_launch("k_update_e<main>");
+  }
+  RegionSet rg = make_set(regs, nreg);
+  if (rg.total > 0) {
+    hipLaunchKernelGGL((k_update_e<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, tf, tinc,
+                       rg);
+    check_launch("k_update_e<edge>");
+  }
+}
+
+void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Tfsf& tf,
+                     double t_inc, const Region* regs, int nreg, const Region& main, hipStream_t st) {
+  if (store == 0) upd_h<0>(g, fields, sc, pm, tf, t_inc, regs, nreg, main, st);
+  else upd_h<1>(g, fields, sc, pm, tf, t_inc, regs, nreg, main, st);
+}
+
+void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                     const Materials& mt, const Ade& ade, const Tfsf& tf, double t_inc, const Region* regs,
+                     int nreg, const Region& main, hipStream_t st) {
+  if (store == 0) upd_e<0>(g, fields, sc, pm, mt, ade, tf, t_inc, regs, nreg, main, st);
+  else upd_e<1>(g, fields, sc, pm, mt, ade, tf, t_inc, regs, nreg, main, st);
+}
+
+void launch_soft(int store, const Grid&, const void* fields, const Scales& sc, const Soft& so, hipStream_t st) {
+  if (so.n <= 0) return;
+  if (store == 0)
+    hipLaunchKernelGGL(k_soft<0>, dim3(1), dim3(64), 0, st, *static_cast<const Fields<0>*>(fields), sc, so);
+  else
+    hipLaunchKernelGGL(k_soft<1>, dim3(1), dim3(64), 0, st, *static_cast<const Fields<1>*>(fields), sc, so);
+  check_launch("k_soft");
+}
+
+void launch_record(int store, const Grid& g, const void* fields, const Scales& sc, const Probes& pr, hipStream_t st) {
+  if (pr.n <= 0) return;
+  const int nb = (pr.n + 63) / 64;
+  if (store == 0)
+    hipLaunchKernelGGL(k_record<0>, dim3(nb), dim3(64), 0, st, g, *static_cast<const Fields<0>*>(fields), sc, pr);
+  else
+    hipLaunchKernelGGL(k_record<1>, dim3(nb), dim3(64), 0, st, g, *static_cast<const Fields<1>*>(fields), sc, pr);
+  check_launch("k_record");
+}
+
+void launch_energy(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt, double* out,
+                   hipStream_t st) {
+  if (store == 0)
+    hipLaunchKernelGGL(k_energy<0>, dim3(1024), dim3(256), 0, st, g, *static_cast<const Fields<0>*>(fields), sc, mt,
+                       out);
+  else
+    hipLaunchKernelGGL(k_energy<1>, dim3(1024), dim3(256), 0, st, g, *static_cast<const Fields<1>*>(fields), sc, mt,
+                       out);
+  check_launch("k_energy");
+}
+
+void launch_to_float(int store, const void* src, float inv, float* dst, long long n, hipStream_t st) {
+  if (store == 0) {
+    (void)hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st);
+    return;
+  }
+  hipLaunchKernelGGL(k_to_float_h, dim3(1024), dim3(256), 0, st, static_cast<const __half*>(src), inv, dst, n);
+  check_launch("k_to_float_h");
+}
+
+void launch_from_float(int store, const float* src, float s, void* dst, long long n, hipStream_t st) {
+  if (store == 0) {
+    (void)hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st);
+    return;
+  }
+  hipLaunchKernelGGL(k_from_float_h, dim3(1024), dim3(256), 0, st, src, s, static_cast<__half*>(dst), n);
+  check_launch("k_from_float_h");
+}
+
+const char* kernel_variant_name(int store) {
+  return store == 0 ? "k_update_{h,e}<fp32>" : "k_update_{h,e}<fp16>";
+}
+
+}  // namespace dtm

@@ -1,0 +1,983 @@
+// dtm_solver.cpp — host side of the C ABI (include/dtmaxwell4.h).
+//
+// One dtm_solver owns the device state of one grid on the HIP device that was
+// current at dtm_create, and one HIP stream (default priority).  All kernel
+// launches are asynchronous on that stream; reads synchronise it.
+//
+// Reference counterparts: GeoParamsHost::set (DTmxw.cu:278-487) for the
+// checks and allocations, init_index / setPMLcoeffs (DTmxw.cu:250-265,
+// 502-517) for the absorbing-layer tables, init_material (DTmxw.cu:604-642)
+// and DispStruct::set (dispersion.cpp:41-82) for the coefficient tables,
+// TFSFsrc (signal.hpp:99-160) for the source, Sensor (sensor.h) for probes.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/dtmaxwell4.h"
+#include "dtm_internal.h"
+
+using namespace dtm;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return fail(DTM_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+struct dtm_solver {
+  dtm_config cfg;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  Grid g{};
+  Scales sc{};
+  int store = 0;
+  void* E[3] = {};
+  void* H[3] = {};
+  Fields<0> f32{};
+  Fields<1> f16{};
+  Cpml pm{};
+  std::vector<void*> allocs;  // everything to free
+  Region main{};
+  Region edge[4]{};
+  int nedge = 0;
+  // materials
+  std::vector<uint8_t> cls_host;
+  std::vector<ClassEntry> classes;
+  std::vector<std::array<uint8_t, 3>> class_mats;
+  uint8_t* cls_dev = nullptr;
+  ClassEntry* table_dev = nullptr;
+  Materials mt{};
+  Ade ade{};
+  AdeMat* ademats_dev = nullptr;
+  GyroMat* gyromats_dev = nullptr;
+  float* design_dev = nullptr;
+  std::vector<float> design_host;
+  // TF/SF
+  Tfsf tf{};
+  // probes
+  std::vector<ProbeDesc> probes;
+  ProbeDesc* probes_dev = nullptr;
+  float* series_dev = nullptr;
+  int series_cap = 0;
+  int series_row = 0;
+  std::vector<float> series_host;
+  int nfreq = 0;
+  double* omega_dev = nullptr;
+  double* dft_dev = nullptr;
+  double* energy_dev = nullptr;
+  float* tmp_dev = nullptr;  // conversion buffer (FP16 field access)
+  long long step = 0;
+  bool started = false;
+  // per-group kernel timing
+  bool timing = false;
+  std::vector<std::array<hipEvent_t, 2>> ev_pool;
+  std::vector<int> ev_group;
+  size_t ev_used = 0;
+  double grp_ms[5] = {};
+  long long grp_cnt[5] = {};
+};
+
+template <typename T>
+static int dalloc(dtm_solver* s, T** p, size_t bytes) {
+  void* q = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(DTM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemsetAsync(q, 0, bytes, s->stream);
+  if (e != hipSuccess) return fail(DTM_ERR_DEVICE, hipGetErrorString(e));
+  s->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return 0;
+}
+
+extern "C" {
+
+const char* dtm_last_error(void) { return g_err.c_str(); }
+const char* dtm_version(void) { return "dtmaxwell4-amd 0.1 (gfx950)"; }
+
+int dtm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void LorenzDisp_set(LorenzDisp* p, double Deps, double wp, double gp, double gp_) {
+  p->a0 = Deps * wp * wp;
+  p->a1 = Deps * gp_;
+  p->b0 = wp * wp;
+  p->b1 = 2 * gp;
+  p->b2 = 1;
+}
+void LorenzDisp_setDrude(LorenzDisp* p, double Deps, double wp, double gp, double gp_) {
+  p->a0 = Deps * wp * wp;
+  p->a1 = Deps * gp_;
+  p->b0 = 0.0;
+  p->b1 = 2 * gp;
+  p->b2 = 1;
+}
+
+// CoffStruct::set_eps (defs.h:96-117): Cayley pair of the static gyration.
+void CoffStruct_set_eps(CoffStruct* c, double eps, int disp, double gx, double gy, double gz, double w, double dt) {
+  c->deps = (float)(1. / eps);
+  c->isDisp = disp;
+  const double ex = eps, ey = eps, ez = eps;
+  gx *= w * dt;
+  gy *= w * dt;
+  gz *= w * dt;
+  const double D = 4 * ex * ey * ez + ex * gx * gx + ey * gy * gy + ez * gz * gz;
+  c->depsXX = (float)((gx * gx + 4 * ey * ez) / D);
+  c->coffAnisoXY = (float)((gx * gy + 2 * ez * gz) / D);
+  c->coffAnisoXZ = (float)((gx * gy - 2 * ey * gy) / D);
+  c->coffAnisoYX = (float)((gx * gy - 2 * ez * gz) / D);
+  c->depsYY = (float)((gy * gy + 4 * ex * ez) / D);
+  c->coffAnisoYZ = (float)((gy * gz + 2 * ex * gx) / D);
+  c->coffAnisoZX = (float)((gx * gz + 2 * ey * gy) / D);
+  c->coffAnisoZY = (float)((gy * gz - 2 * ex * gx) / D);
+  c->depsZZ = (float)((gz * gz + 4 * ex * ey) / D);
+  c->gyrationXX = (float)((4 * ex * ey * ez + ex * gx * gx - ey * gy * gy - ez * gz * gz) / D);
+  c->gyrationXY = (float)((4 * ey * ez * gz + 2 * ey * gx * gy) / D);
+  c->gyrationXZ = (float)((-4 * ey * ez * gy + 2 * ez * gx * gz) / D);
+  c->gyrationYX = (float)((-4 * ex * ez * gz + 2 * ex * gx * gy) / D);
+  c->gyrationYY = (float)((4 * ex * ey * ez + ey * gy * gy - ez * gz * gz - ex * gx * gx) / D);
+  c->gyrationYZ = (float)((4 * ex * ez * gx + 2 * ez * gy * gz) / D);
+  c->gyrationZX = (float)((4 * ex * ey * gy + 2 * ex * gx * gz) / D);
+  c->gyrationZY = (float)((-4 * ex * ey * gx + 2 * ey * gy * gz) / D);
+  c->gyrationZZ = (float)((4 * ex * ey * ez + ez * gz * gz - ex * gx * gx - ey * gy * gy) / D);
+}
+
+void dtm_config_defaults(dtm_config* c, int nx, int ny, int nz) {
+  memset(c, 0, sizeof(*c));
+  c->nx = nx;
+  c->ny = ny;
+  c->nz = nz;
+  c->dx = c->dy = c->dz = 0.005;  // py_consts.h:71
+  c->dt = 0.002;
+  c->pml_order = 4.0;  // DTmxw.cu:224
+  c->pml_R = 1e-4;     // DTmxw.cu:225 (log(10000))
+  c->pml_vmax = 1.0;
+  c->nmat = 1;
+  c->mat[0].eps_inf = 1.0;
+  c->tfsf.amp = 1.0;
+  c->tfsf.cSrc = 1.0;
+  c->tfsf.Imp = 1.0;
+  c->tfsf.sf_inside = 1;
+  c->ntime = 1;
+}
+
+}  // extern "C"
+
+// sigma at half-coordinate h (same grading as DTmxw.cu:220-249 with the
+// standard CPML depth measured from the inner face of the slab).
+static double pml_sigma(int h, int n, int plo, int phi, double d, double m, double R, double vmax) {
+  if (plo > 0 && h < 2 * plo) {
+    const double L = plo * d;
+    const double smax = vmax * log(1.0 / R) * (m + 1) / (2 * L);
+    return smax * pow((double)(2 * plo - h) / (2.0 * plo), m);
+  }
+  if (phi > 0 && h > 2 * (n - phi)) {
+    const double L = phi * d;
+    const double smax = vmax * log(1.0 / R) * (m + 1) / (2 * L);
+    return smax * pow((double)(h - 2 * (n - phi)) / (2.0 * phi), m);
+  }
+  return 0.0;
+}
+
+static void ade_coeffs(const dtm_material& m, double dt, AdeMat& out) {
+  // DispStruct::set (dispersion.cpp:41-82)
+  double zm = 0, zp = 0, z0 = 0;
+  double al[kMaxPoles], xi[kMaxPoles], zetp[kMaxPoles], zetm[kMaxPoles], zet[kMaxPoles];
+  const int np = std::min(m.npoles, kMaxPoles);
+  for (int i = 0; i < np; i++) {
+    const LorenzDisp& L = m.pole[i];
+    const double den = L.b1 * dt + 2;
+    al[i] = (4 - 2 * L.b0 * dt * dt) / den;
+    xi[i] = (L.b1 * dt - 2) / den;
+    zetp[i] = (L.a0 * dt * dt + 2 * L.a1 * dt) / den;
+    zetm[i] = (-L.a0 * dt * dt + 2 * L.a1 * dt) / den;
+    zet[i] = -(4 * L.a1 * dt) / den;
+    zm += zetm[i];
+    zp += zetp[i];
+    z0 += zet[i];
+  }
+  const double D = 2 * m.eps_inf + m.sigma * dt + zp;
+  const double C1 = -zm / D, C2 = (2 * m.eps_inf - m.sigma * dt - z0) / D, C3 = 2 * dt / D;
+  out.np = np;
+  out.kE[0] = (float)C1;
+  out.kE[1] = (float)C2;
+  out.kE[2] = (float)(C3 / dt);
+  for (int i = 0; i < np; i++) {
+    out.kEJ[i][0] = (float)(C3 * 0.5 * (1 + al[i]));
+    out.kEJ[i][1] = (float)(C3 * 0.5 * xi[i]);
+    out.kJ[i][0] = (float)al[i];
+    out.kJ[i][1] = (float)xi[i];
+    out.kJ[i][2] = (float)(zetp[i] / dt);
+    out.kJ[i][3] = (float)(zetm[i] / dt);
+    out.kJ[i][4] = (float)(zet[i] / dt);
+  }
+}
+
+// Magnetised-plasma current update matrices (trapezoidal rotation, no
+// reference counterpart: the reference's Gyration() is disabled).
+static void gyro_coeffs(const dtm_material& m, double dt, GyroMat& out) {
+  const double w[3] = {m.gyro_wc[0], m.gyro_wc[1], m.gyro_wc[2]};
+  const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  double M[9], N[9];
+  for (int r = 0; r < 3; r++)
+    for (int q = 0; q < 3; q++) {
+      const double I = r == q ? 1.0 : 0.0;
+      M[3 * r + q] = I * (1 + 0.5 * m.gyro_nu * dt) - 0.5 * dt * W[3 * r + q];
+      N[3 * r + q] = I * (1 - 0.5 * m.gyro_nu * dt) + 0.5 * dt * W[3 * r + q];
+    }
+  const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                     M[2] * (M[3] * M[7] - M[4] * M[6]);
+  double Mi[9];
+  Mi[0] = (M[4] * M[8] - M[5] * M[7]) / det;
+  Mi[1] = (M[2] * M[7] - M[1] * M[8]) / det;
+  Mi[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+  Mi[3] = (M[5] * M[6] - M[3] * M[8]) / det;
+  Mi[4] = (M[0] * M[8] - M[2] * M[6]) / det;
+  Mi[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+  Mi[6] = (M[3] * M[7] - M[4] * M[6]) / det;
+  Mi[7] = (M[1] * M[6] - M[0] * M[7]) / det;
+  Mi[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+  for (int r = 0; r < 3; r++)
+    for (int q = 0; q < 3; q++) {
+      double a = 0;
+      for (int t = 0; t < 3; t++) a += Mi[3 * r + t] * N[3 * t + q];
+      out.A[3 * r + q] = (float)a;
+      out.B[3 * r + q] = (float)(dt * m.gyro_wp * m.gyro_wp * Mi[3 * r + q]);
+    }
+}
+
+static double wave_value(int waveform, double w, double kEnv, double t0, double tw, double t) {
+  const double p = w * t;
+  if (waveform == DTM_WAVE_RAISED_COS) {
+    const double kp = p * kEnv;
+    if (kp < 0 || kp > 2 * M_PI) return 0;
+    return sin(p) * 0.5 * (1 - cos(kp));
+  }
+  const double u = (p / w - t0) / tw;
+  return exp(-u * u) * sin(p);
+}
+
+// inSF bounds on half-coordinates: inside <=> h*d/2 > BoxM && h*d/2 < BoxP,
+// evaluated with the same double expression as signal.hpp:154-157.
+static void sf_bounds(double bm, double bp, double d, int nh, int& lo, int& hi) {
+  lo = 1 << 30;
+  hi = -(1 << 30);
+  for (int h = -8; h <= nh + 8; h++) {
+    const double x = h * 0.5 * d;
+    if (x > bm && x < bp) {
+      lo = std::min(lo, h);
+      hi = std::max(hi, h);
+    }
+  }
+}
+
+static int setup_tfsf(dtm_solver* s) {
+  const dtm_config& c = s->cfg;
+  Tfsf& t = s->tf;
+  memset(&t, 0, sizeof(t));
+  t.kind = c.tfsf.kind;
+  if (t.kind == DTM_TFSF_NONE) return 0;
+  if (t.kind != DTM_TFSF_DIPOLE && t.kind != DTM_TFSF_PLANE) return fail(DTM_ERR_ARG, "unknown tfsf kind");
+  t.waveform = c.tfsf.waveform;
+  t.sf_inside = c.tfsf.sf_inside;
+  const int n[3] = {c.nx, c.ny, c.nz};
+  const double d[3] = {c.dx, c.dy, c.dz};
+  for (int a = 0; a < 3; a++) {
+    sf_bounds(c.tfsf.box[2 * a], c.tfsf.box[2 * a + 1], d[a], 2 * n[a], t.hlo[a], t.hhi[a]);
+    t.half_d[a] = 0.5 * d[a];
+    if (t.hlo[a] > t.hhi[a]) {  // empty box: no face anywhere
+      t.hlo[a] = 1 << 29;
+      t.hhi[a] = -(1 << 29);
+      t.clo[a] = 1;
+      t.chi[a] = 0;
+      continue;
+    }
+    // a neighbour at half-offset within +-3 can cross a face only near it
+    t.clo[a] = (t.hlo[a] - 4) / 2 - 1;
+    t.chi[a] = (t.hhi[a] + 4) / 2 + 1;
+    t.fl0[a] = (t.hlo[a] - 4) / 2 - 1;
+    t.fl1[a] = (t.hlo[a] + 3) / 2 + 1;
+    t.fh0[a] = (t.hhi[a] - 4) / 2 - 1;
+    t.fh1[a] = (t.hhi[a] + 4) / 2 + 1;
+  }
+  t.w = c.tfsf.w;
+  t.kEnv = c.tfsf.kEnv;
+  t.cSrc = c.tfsf.cSrc;
+  t.Imp = c.tfsf.Imp;
+  t.t0 = c.tfsf.t0;
+  t.tw = c.tfsf.tw;
+  t.amp = c.tfsf.amp;
+  t.st = sin(c.tfsf.theta);
+  t.ct = cos(c.tfsf.theta);
+  t.sph = sin(c.tfsf.phi);
+  t.cph = cos(c.tfsf.phi);
+  t.sps = sin(c.tfsf.psi);
+  t.cps = cos(c.tfsf.psi);
+  const double k = c.tfsf.w / c.tfsf.cSrc;
+  t.kx = k * t.st * t.cph;
+  t.ky = k * t.st * t.sph;
+  t.kz = k * t.ct;
+  for (int a = 0; a < 3; a++) {
+    t.pos[a] = c.tfsf.pos[a];
+    t.dip[a] = 0.5 * (c.tfsf.box[2 * a] + c.tfsf.box[2 * a + 1]);
+  }
+  return 0;
+}
+
+static bool tfsf_gate(const dtm_solver* s, long long n) {
+  const dtm_config& c = s->cfg;
+  if (c.tfsf.kind == DTM_TFSF_NONE) return false;
+  const int nt = c.ntime > 0 ? c.ntime : 1;
+  const long long istep = n / nt;
+  return (double)((istep + 1) * nt) * c.dt < c.tfsf.tStop;  // window.hpp:125
+}
+
+static int setup_cpml(dtm_solver* s) {
+  const dtm_config& c = s->cfg;
+  Cpml& p = s->pm;
+  const int n[3] = {c.nx, c.ny, c.nz};
+  const double d[3] = {c.dx, c.dy, c.dz};
+  p.on = 0;
+  for (int a = 0; a < 3; a++) {
+    p.plo[a] = c.npml[2 * a];
+    p.phi[a] = c.npml[2 * a + 1];
+    if (p.plo[a] > 0 || p.phi[a] > 0) p.on = 1;
+  }
+  for (int a = 0; a < 3; a++) {
+    std::vector<float> bE(n[a]), cE(n[a]), bH(n[a]), cH(n[a]);
+    for (int v = 0; v < n[a]; v++) {
+      const double se = pml_sigma(2 * v + 1, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax);
+      const double sh = pml_sigma(2 * v, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax);
+      const double be = exp(-se * c.dt), bh = exp(-sh * c.dt);
+      bE[v] = (float)be;
+      cE[v] = (float)(be - 1.0);
+      bH[v] = (float)bh;
+      cH[v] = (float)(bh - 1.0);
+    }
+    float* t[4];
+    for (int q = 0; q < 4; q++)
+      if (int rc = dalloc(s, &t[q], sizeof(float) * n[a])) return rc;
+    HIPCHK(hipMemcpy(t[0], bE.data(), sizeof(float) * n[a], hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t[1], cE.data(), sizeof(float) * n[a], hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t[2], bH.data(), sizeof(float) * n[a], hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t[3], cH.data(), sizeof(float) * n[a], hipMemcpyHostToDevice));
+    p.bE[a] = t[0];
+    p.cE[a] = t[1];
+    p.bH[a] = t[2];
+    p.cH[a] = t[3];
+    const int P = p.plo[a] + p.phi[a];
+    size_t cnt = 0;
+    if (P > 0) cnt = (size_t)P * (a == 0 ? (size_t)c.ny * c.nz : a == 1 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny);
+    for (int sl = 0; sl < 2; sl++) {
+      if (int rc = dalloc(s, &p.psiE[a][sl], sizeof(float) * cnt)) return rc;
+      if (int rc = dalloc(s, &p.psiH[a][sl], sizeof(float) * cnt)) return rc;
+    }
+  }
+  return 0;
+}
+
+// Interior box: every x/y stencil neighbour exists and no x/y CPML; edges
+// take the rest with full boundary handling.
+static void setup_regions(dtm_solver* s) {
+  const dtm_config& c = s->cfg;
+  auto edge = [&](int a, int lo_or_hi) {
+    const int p = c.npml[2 * a + lo_or_hi];
+    const int per = c.bc[a] == DTM_BC_PERIODIC;
+    return std::max(p, per || true ? 2 : 0);
+  };
+  int x0 = std::min(edge(0, 0), c.nx), x1 = std::max(c.nx - edge(0, 1), x0);
+  int y0 = std::min(edge(1, 0), c.ny), y1 = std::max(c.ny - edge(1, 1), y0);
+  s->main = Region{x0, x1, y0, y1, 0, c.nz};
+  s->nedge = 0;
+  s->edge[s->nedge++] = Region{0, x0, 0, c.ny, 0, c.nz};
+  s->edge[s->nedge++] = Region{x1, c.nx, 0, c.ny, 0, c.nz};
+  s->edge[s->nedge++] = Region{x0, x1, 0, y0, 0, c.nz};
+  s->edge[s->nedge++] = Region{x0, x1, y1, c.ny, 0, c.nz};
+}
+
+static void field_structs(dtm_solver* s) {
+  for (int c = 0; c < 3; c++) {
+    s->f32.E[c] = static_cast<float*>(s->E[c]);
+    s->f32.H[c] = static_cast<float*>(s->H[c]);
+    s->f16.E[c] = static_cast<__half*>(s->E[c]);
+    s->f16.H[c] = static_cast<__half*>(s->H[c]);
+  }
+}
+
+static const void* fields_ptr(const dtm_solver* s) {
+  return s->store == 0 ? static_cast<const void*>(&s->f32) : static_cast<const void*>(&s->f16);
+}
+
+// Build the class table from per-component material ids.
+static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
+  const dtm_config& c = s->cfg;
+  const size_t n = (size_t)c.nx * c.ny * c.nz;
+  std::unordered_map<uint32_t, int> ids;
+  s->classes.clear();
+  s->class_mats.clear();
+  s->cls_host.assign(n, 0);
+  auto entry_for = [&](uint8_t a, uint8_t b, uint8_t d) -> int {
+    const uint32_t key = a | (b << 8) | (d << 16);
+    auto it = ids.find(key);
+    if (it != ids.end()) return it->second;
+    const int id = (int)s->classes.size();
+    if (id >= kMaxClasses) return -1;
+    ids[key] = id;
+    ClassEntry ce{};
+    const uint8_t mm[3] = {a, b, d};
+    for (int q = 0; q < 3; q++) {
+      const dtm_material& M = c.mat[mm[q]];
+      ce.inv_eps[q] = (float)(1.0 / (M.eps_inf > 0 ? M.eps_inf : 1.0));
+      ce.mat[q] = mm[q];
+      if (M.npoles > 0 || M.sigma != 0) ce.special |= (uint8_t)(1 << q);
+    }
+    if (c.mat[a].gyro) ce.special |= 8;
+    s->classes.push_back(ce);
+    s->class_mats.push_back({a, b, d});
+    return id;
+  };
+  entry_for(0, 0, 0);
+  int bb[6] = {c.nx, -1, c.ny, -1, c.nz, -1};
+  bool any_special = false, any_gyro = false;
+  for (size_t q = 0; q < n; q++) {
+    const uint8_t a = m[0] ? m[0][q] : 0, b = m[1] ? m[1][q] : 0, d = m[2] ? m[2][q] : 0;
+    if (a >= c.nmat || b >= c.nmat || d >= c.nmat) return fail(DTM_ERR_ARG, "material id >= nmat");
+    const int id = entry_for(a, b, d);
+    if (id < 0) return fail(DTM_ERR_LIMIT, "more than 256 distinct (Ex,Ey,Ez) material classes");
+    s->cls_host[q] = (uint8_t)id;
+    if (s->classes[id].special) {
+      any_special = true;
+      if (s->classes[id].special & 8) any_gyro = true;
+      const int i = (int)(q / ((size_t)c.ny * c.nz)), j = (int)((q / c.nz) % c.ny), k = (int)(q % c.nz);
+      bb[0] = std::min(bb[0], i);
+      bb[1] = std::max(bb[1], i);
+      bb[2] = std::min(bb[2], j);
+      bb[3] = std::max(bb[3], j);
+      bb[4] = std::min(bb[4], k);
+      bb[5] = std::max(bb[5], k);
+    }
+  }
+  const int ncls = (int)s->classes.size();
+  if (!s->table_dev)
+    if (int rc = dalloc(s, &s->table_dev, sizeof(ClassEntry) * kMaxClasses)) return rc;
+  HIPCHK(hipMemcpy(s->table_dev, s->classes.data(), sizeof(ClassEntry) * ncls, hipMemcpyHostToDevice));
+  if (ncls > 1) {
+    if (!s->cls_dev)
+      if (int rc = dalloc(s, &s->cls_dev, n)) return rc;
+    HIPCHK(hipMemcpy(s->cls_dev, s->cls_host.data(), n, hipMemcpyHostToDevice));
+  }
+  s->mt.cls = ncls > 1 ? s->cls_dev : nullptr;
+  s->mt.table = s->table_dev;
+  s->mt.nclass = ncls;
+  // dispersive / gyrotropic state over the bounding box of special cells
+  Ade& A = s->ade;
+  A.on = any_special ? 1 : 0;
+  A.gyro_on = any_gyro ? 1 : 0;
+  if (any_special) {
+    A.i0 = bb[0];
+    A.j0 = bb[2];
+    A.k0 = bb[4];
+    A.ni = bb[1] - bb[0] + 1;
+    A.nj = bb[3] - bb[2] + 1;
+    A.nk = bb[5] - bb[4] + 1;
+    A.bsy = A.nk;
+    A.bsx = (long long)A.nj * A.nk;
+    const size_t nb = (size_t)A.ni * A.nj * A.nk;
+    int maxp = 0;
+    std::vector<AdeMat> am(DTM_MAX_MATS);
+    std::vector<GyroMat> gm(DTM_MAX_MATS);
+    for (int mm = 0; mm < c.nmat; mm++) {
+      ade_coeffs(c.mat[mm], c.dt, am[mm]);
+      maxp = std::max(maxp, am[mm].np);
+      if (c.mat[mm].gyro) gyro_coeffs(c.mat[mm], c.dt, gm[mm]);
+    }
+    for (int q = 0; q < 3; q++) {
+      if (int rc = dalloc(s, &A.Em[q], sizeof(float) * nb)) return rc;
+      for (int p = 0; p < maxp; p++) {
+        if (int rc = dalloc(s, &A.Jp[q][p], sizeof(float) * nb)) return rc;
+        if (int rc = dalloc(s, &A.Jm[q][p], sizeof(float) * nb)) return rc;
+      }
+      if (any_gyro)
+        if (int rc = dalloc(s, &A.Jg[q], sizeof(float) * nb)) return rc;
+    }
+    if (int rc = dalloc(s, &s->ademats_dev, sizeof(AdeMat) * DTM_MAX_MATS)) return rc;
+    if (int rc = dalloc(s, &s->gyromats_dev, sizeof(GyroMat) * DTM_MAX_MATS)) return rc;
+    HIPCHK(hipMemcpy(s->ademats_dev, am.data(), sizeof(AdeMat) * DTM_MAX_MATS, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->gyromats_dev, gm.data(), sizeof(GyroMat) * DTM_MAX_MATS, hipMemcpyHostToDevice));
+    A.mats = s->ademats_dev;
+    A.gmats = s->gyromats_dev;
+  }
+  // soft sources must sit in plain cells
+  for (int q = 0; q < c.nsoft; q++) {
+    const dtm_soft_source& so = c.soft[q];
+    const size_t idx = ((size_t)so.i * c.ny + so.j) * c.nz + so.k;
+    if (s->classes[s->cls_host[idx]].special)
+      return fail(DTM_ERR_ARG, "soft source placed in a dispersive/gyrotropic cell");
+  }
+  return 0;
+}
+
+extern "C" {
+
+int dtm_create(const dtm_config* cfg, dtm_solver** out) {
+  if (!cfg || !out) return fail(DTM_ERR_ARG, "null argument");
+  *out = nullptr;
+  const dtm_config& c = *cfg;
+  if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(DTM_ERR_ARG, "grid sizes must be positive");
+  if ((long long)c.nx * c.ny * c.nz > (1LL << 34)) return fail(DTM_ERR_LIMIT, "grid too large");
+  if (!(c.dx > 0 && c.dy > 0 && c.dz > 0 && c.dt > 0)) return fail(DTM_ERR_ARG, "steps must be positive");
+  const double cour = c.dt * sqrt(1 / (c.dx * c.dx) + 1 / (c.dy * c.dy) + 1 / (c.dz * c.dz));
+  if (cour > 6. / 7.) return fail(DTM_ERR_COURANT, "Courant condition is not satisfied (DTmxw.cu:332)");
+  if (c.nmat < 1 || c.nmat > DTM_MAX_MATS) return fail(DTM_ERR_ARG, "nmat out of range");
+  if (c.nsoft < 0 || c.nsoft > DTM_MAX_SOFT) return fail(DTM_ERR_ARG, "nsoft out of range");
+  if (c.storage != DTM_STORE_FP32 && c.storage != DTM_STORE_FP16) return fail(DTM_ERR_ARG, "bad storage");
+  const int n3[3] = {c.nx, c.ny, c.nz};
+  for (int a = 0; a < 3; a++) {
+    if (c.npml[2 * a] < 0 || c.npml[2 * a + 1] < 0 || c.npml[2 * a] + c.npml[2 * a + 1] > n3[a])
+      return fail(DTM_ERR_ARG, "CPML thicker than the grid");
+    if (c.bc[a] == DTM_BC_PERIODIC && (c.npml[2 * a] || c.npml[2 * a + 1]))
+      return fail(DTM_ERR_ARG, "CPML on a periodic axis");
+    if (c.bc[a] == DTM_BC_PERIODIC && n3[a] < 4) return fail(DTM_ERR_ARG, "periodic axis needs >= 4 cells");
+  }
+  for (int m = 0; m < c.nmat; m++) {
+    if (!(c.mat[m].eps_inf > 0)) return fail(DTM_ERR_ARG, "eps_inf must be positive");
+    if (c.mat[m].npoles < 0 || c.mat[m].npoles > DTM_MAX_POLES) return fail(DTM_ERR_ARG, "npoles out of range");
+  }
+  for (int q = 0; q < c.nsoft; q++) {
+    const dtm_soft_source& so = c.soft[q];
+    if (so.comp < 0 || so.comp > 2 || so.i < 0 || so.i >= c.nx || so.j < 0 || so.j >= c.ny || so.k < 0 ||
+        so.k >= c.nz)
+      return fail(DTM_ERR_ARG, "soft source outside the grid");
+  }
+  dtm_solver* s = new dtm_solver();
+  s->cfg = c;
+  s->store = c.storage;
+  hipError_t e = hipGetDevice(&s->dev);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(DTM_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+  }
+  e = hipStreamCreate(&s->stream);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(DTM_ERR_DEVICE, hipGetErrorString(e));
+  }
+  Grid& g = s->g;
+  g.nx = c.nx;
+  g.ny = c.ny;
+  g.nz = c.nz;
+  g.sy = c.nz;
+  g.sx = (long long)c.ny * c.nz;
+  for (int a = 0; a < 3; a++) g.per[a] = c.bc[a] == DTM_BC_PERIODIC;
+  g.rd[0] = (float)(1.0 / (24.0 * c.dx));
+  g.rd[1] = (float)(1.0 / (24.0 * c.dy));
+  g.rd[2] = (float)(1.0 / (24.0 * c.dz));
+  g.dt = (float)c.dt;
+  for (int a = 0; a < 3; a++) {
+    const int se = s->store ? c.scale_e_log2[a] : 0, sh = s->store ? c.scale_h_log2[a] : 0;
+    s->sc.se[a] = ldexpf(1.f, se);
+    s->sc.ie[a] = ldexpf(1.f, -se);
+    s->sc.sh[a] = ldexpf(1.f, sh);
+    s->sc.ih[a] = ldexpf(1.f, -sh);
+  }
+  const size_t n = (size_t)c.nx * c.ny * c.nz;
+  const size_t esz = s->store ? 2 : 4;
+  int rc = 0;
+  for (int q = 0; q < 3 && !rc; q++) {
+    rc = dalloc(s, &s->E[q], esz * n);
+    if (!rc) rc = dalloc(s, &s->H[q], esz * n);
+  }
+  if (!rc) rc = setup_cpml(s);
+  if (!rc) rc = setup_tfsf(s);
+  if (!rc) {
+    const uint8_t* none[3] = {nullptr, nullptr, nullptr};
+    rc = build_classes(s, none);
+  }
+  if (!rc) rc = dalloc(s, &s->energy_dev, sizeof(double));
+  if (rc) {
+    dtm_destroy(s);
+    return rc;
+  }
+  field_structs(s);
+  setup_regions(s);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) {
+    dtm_destroy(s);
+    return fail(DTM_ERR_DEVICE, "stream sync failed after allocation");
+  }
+  *out = s;
+  return 0;
+}
+
+void dtm_destroy(dtm_solver* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& e : s->ev_pool) {
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+  }
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int dtm_set_materials(dtm_solver* s, const uint8_t* mx, const uint8_t* my, const uint8_t* mz) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  if (s->started) return fail(DTM_ERR_STATE, "materials must be set before the first step");
+  const uint8_t* m[3] = {mx, my, mz};
+  return build_classes(s, m);
+}
+
+int dtm_set_design_eps(dtm_solver* s, int i0, int j0, int k0, int ni, int nj, int nk, const double* eps) {
+  if (!s || !eps) return fail(DTM_ERR_ARG, "null argument");
+  const dtm_config& c = s->cfg;
+  if (i0 < 0 || j0 < 0 || k0 < 0 || ni < 1 || nj < 1 || nk < 1 || i0 + ni > c.nx || j0 + nj > c.ny || k0 + nk > c.nz)
+    return fail(DTM_ERR_ARG, "design box outside the grid");
+  const size_t nb = (size_t)ni * nj * nk;
+  if (s->mt.d_on && (s->mt.d_ni != ni || s->mt.d_nj != nj || s->mt.d_nk != nk))
+    return fail(DTM_ERR_ARG, "design box shape cannot change");
+  for (int i = 0; i < ni; i++)
+    for (int j = 0; j < nj; j++)
+      for (int k = 0; k < nk; k++) {
+        const size_t q = ((size_t)(i0 + i) * c.ny + (j0 + j)) * c.nz + (k0 + k);
+        const ClassEntry& ce = s->classes[s->cls_host[q]];
+        if (ce.special) return fail(DTM_ERR_ARG, "design box overlaps a dispersive cell");
+      }
+  s->design_host.resize(nb);
+  for (size_t q = 0; q < nb; q++) {
+    if (!(eps[q] > 0)) return fail(DTM_ERR_ARG, "design eps must be positive");
+    s->design_host[q] = (float)(1.0 / eps[q]);
+  }
+  if (!s->design_dev)
+    if (int rc = dalloc(s, &s->design_dev, sizeof(float) * nb)) return rc;
+  HIPCHK(hipMemcpyAsync(s->design_dev, s->design_host.data(), sizeof(float) * nb, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->mt.d_on = 1;
+  s->mt.d_i0 = i0;
+  s->mt.d_j0 = j0;
+  s->mt.d_k0 = k0;
+  s->mt.d_ni = ni;
+  s->mt.d_nj = nj;
+  s->mt.d_nk = nk;
+  s->mt.d_inv_eps = s->design_dev;
+  return 0;
+}
+
+static int add_probe_desc(dtm_solver* s, const ProbeDesc& d) {
+  if (s->started) return fail(DTM_ERR_STATE, "probes must be added before the first step");
+  if ((int)s->probes.size() >= kMaxProbes) return fail(DTM_ERR_LIMIT, "too many probes");
+  s->probes.push_back(d);
+  return (int)s->probes.size() - 1;
+}
+
+int dtm_add_sensor(dtm_solver* s, const char* fld, int x, int y, int z) {
+  if (!s || !fld) return fail(DTM_ERR_ARG, "null argument");
+  static const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+  int comp = -1;
+  for (int q = 0; q < 6; q++)
+    if (strcmp(fld, names[q]) == 0) comp = q;
+  if (comp < 0) return fail(DTM_ERR_ARG, "sensor field must be one of Ex Ey Ez Hx Hy Hz");
+  const dtm_config& c = s->cfg;
+  if (x < 0 || x >= c.nx || y < 0 || y >= c.ny || z < 0 || z >= c.nz) return fail(DTM_ERR_ARG, "sensor outside grid");
+  return add_probe_desc(s, ProbeDesc{1, comp, x, y, z});
+}
+
+int dtm_add_probe(dtm_solver* s, int comp, int i, int j, int k) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  const dtm_config& c = s->cfg;
+  if (comp < 0 || comp > 5 || i < 0 || i >= c.nx || j < 0 || j >= c.ny || k < 0 || k >= c.nz)
+    return fail(DTM_ERR_ARG, "probe outside grid");
+  return add_probe_desc(s, ProbeDesc{0, comp, i, j, k});
+}
+
+int dtm_set_dft(dtm_solver* s, int nfreq, const double* omegas) {
+  if (!s || nfreq < 0 || (nfreq > 0 && !omegas)) return fail(DTM_ERR_ARG, "bad dft arguments");
+  if (s->started) return fail(DTM_ERR_STATE, "DFT must be configured before the first step");
+  if (nfreq > 4096) return fail(DTM_ERR_LIMIT, "too many DFT frequencies");
+  s->nfreq = nfreq;
+  if (nfreq == 0) return 0;
+  if (int rc = dalloc(s, &s->omega_dev, sizeof(double) * nfreq)) return rc;
+  HIPCHK(hipMemcpy(s->omega_dev, omegas, sizeof(double) * nfreq, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int start(dtm_solver* s) {
+  if (s->started) return 0;
+  const int np = (int)s->probes.size();
+  if (np > 0) {
+    if (int rc = dalloc(s, &s->probes_dev, sizeof(ProbeDesc) * np)) return rc;
+    HIPCHK(hipMemcpy(s->probes_dev, s->probes.data(), sizeof(ProbeDesc) * np, hipMemcpyHostToDevice));
+    s->series_cap = std::max(16, std::min(4096, (1 << 22) / np));
+    if (int rc = dalloc(s, &s->series_dev, sizeof(float) * (size_t)s->series_cap * np)) return rc;
+    if (s->nfreq > 0)
+      if (int rc = dalloc(s, &s->dft_dev, sizeof(double) * 2 * (size_t)np * s->nfreq)) return rc;
+  }
+  s->started = true;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+static int flush_series(dtm_solver* s) {
+  const int np = (int)s->probes.size();
+  if (np == 0 || s->series_row == 0) return 0;
+  const size_t old = s->series_host.size();
+  s->series_host.resize(old + (size_t)s->series_row * np);
+  HIPCHK(hipMemcpyAsync(s->series_host.data() + old, s->series_dev, sizeof(float) * (size_t)s->series_row * np,
+                        hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->series_row = 0;
+  return 0;
+}
+
+static const Region kEmpty{0, 0, 0, 0, 0, 0};
+
+static void ev_begin(dtm_solver* s, int group) {
+  if (!s->timing) return;
+  if (s->ev_used == s->ev_pool.size()) {
+    std::array<hipEvent_t, 2> e{};
+    (void)hipEventCreate(&e[0]);
+    (void)hipEventCreate(&e[1]);
+    s->ev_pool.push_back(e);
+    s->ev_group.push_back(0);
+  }
+  s->ev_group[s->ev_used] = group;
+  (void)hipEventRecord(s->ev_pool[s->ev_used][0], s->stream);
+}
+static void ev_end(dtm_solver* s) {
+  if (!s->timing) return;
+  (void)hipEventRecord(s->ev_pool[s->ev_used][1], s->stream);
+  s->ev_used++;
+}
+
+static int one_step(dtm_solver* s) {
+  const dtm_config& c = s->cfg;
+  const long long n = s->step;
+  s->tf.active = tfsf_gate(s, n) ? 1 : 0;
+  const void* F = fields_ptr(s);
+  const double th = (double)n * c.dt, te = ((double)n + 0.5) * c.dt;
+  // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
+  ev_begin(s, 0);
+  launch_update_h(s->store, s->g, F, s->sc, s->pm, s->tf, th, nullptr, 0, s->main, s->stream);
+  ev_end(s);
+  ev_begin(s, 1);
+  launch_update_h(s->store, s->g, F, s->sc, s->pm, s->tf, th, s->edge, s->nedge, kEmpty, s->stream);
+  ev_end(s);
+  // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
+  ev_begin(s, 2);
+  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, te, nullptr, 0, s->main, s->stream);
+  ev_end(s);
+  ev_begin(s, 3);
+  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, te, s->edge, s->nedge, kEmpty, s->stream);
+  ev_end(s);
+  ev_begin(s, 4);
+  if (c.nsoft > 0) {
+    Soft so{};
+    so.n = c.nsoft;
+    const double th = ((double)n + 0.5) * c.dt;
+    for (int q = 0; q < c.nsoft; q++) {
+      const dtm_soft_source& ss = c.soft[q];
+      const size_t idx = ((size_t)ss.i * c.ny + ss.j) * c.nz + ss.k;
+      const ClassEntry& ce = s->classes[s->cls_host[idx]];
+      double ie = ce.inv_eps[ss.comp];
+      if (s->mt.d_on) {
+        const int di = ss.i - s->mt.d_i0, dj = ss.j - s->mt.d_j0, dk = ss.k - s->mt.d_k0;
+        if (di >= 0 && di < s->mt.d_ni && dj >= 0 && dj < s->mt.d_nj && dk >= 0 && dk < s->mt.d_nk)
+          ie = s->design_host[((size_t)di * s->mt.d_nj + dj) * s->mt.d_nk + dk];
+      }
+      const double J = ss.amp * wave_value(ss.waveform, ss.w, ss.kEnv, ss.t0, ss.tw, th);
+      so.comp[q] = ss.comp;
+      so.q[q] = (long long)idx;
+      so.val[q] = (float)(c.dt * ie * J);
+    }
+    launch_soft(s->store, s->g, F, s->sc, so, s->stream);
+  }
+  const int np = (int)s->probes.size();
+  if (np > 0) {
+    Probes pr{};
+    pr.n = np;
+    pr.desc = s->probes_dev;
+    pr.series = s->series_dev;
+    pr.row = s->series_row;
+    pr.nfreq = s->nfreq;
+    pr.omega = s->omega_dev;
+    pr.dft = s->dft_dev;
+    pr.dt = c.dt;
+    pr.step = n;
+    launch_record(s->store, s->g, F, s->sc, pr, s->stream);
+  }
+  ev_end(s);
+  if (np > 0) {
+    s->series_row++;
+    if (s->series_row == s->series_cap)
+      if (int rc = flush_series(s)) return rc;
+  }
+  s->step++;
+  return 0;
+}
+
+int dtm_step(dtm_solver* s, int nsteps) {
+  if (!s || nsteps < 0) return fail(DTM_ERR_ARG, "bad step arguments");
+  if (int rc = start(s)) return rc;
+  for (int t = 0; t < nsteps; t++)
+    if (int rc = one_step(s)) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(DTM_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+long long dtm_time_step(const dtm_solver* s) { return s ? s->step : -1; }
+
+int dtm_sync(dtm_solver* s) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int dtm_probe_count(const dtm_solver* s) { return s ? (int)s->probes.size() : 0; }
+
+int dtm_probe_read(dtm_solver* s, float* out, int nsteps_max) {
+  if (!s || (!out && nsteps_max > 0)) return fail(DTM_ERR_ARG, "null argument");
+  if (int rc = flush_series(s)) return rc;
+  const int np = (int)s->probes.size();
+  if (np == 0) return 0;
+  const long long rows = (long long)(s->series_host.size() / np);
+  const long long r = std::min<long long>(rows, nsteps_max);
+  memcpy(out, s->series_host.data(), sizeof(float) * (size_t)r * np);
+  return (int)r;
+}
+
+int dtm_dft_read(dtm_solver* s, double* out) {
+  if (!s || !out) return fail(DTM_ERR_ARG, "null argument");
+  const size_t cnt = 2 * s->probes.size() * (size_t)s->nfreq;
+  if (cnt == 0) return 0;
+  if (!s->dft_dev) {
+    memset(out, 0, sizeof(double) * cnt);
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(out, s->dft_dev, sizeof(double) * cnt, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+static void* comp_ptr(dtm_solver* s, int comp) { return comp < 3 ? s->E[comp] : s->H[comp - 3]; }
+
+int dtm_field_read(dtm_solver* s, int comp, float* buf) {
+  if (!s || !buf || comp < 0 || comp > 5) return fail(DTM_ERR_ARG, "bad field read arguments");
+  const size_t n = (size_t)s->cfg.nx * s->cfg.ny * s->cfg.nz;
+  if (s->store == 0) {
+    HIPCHK(hipMemcpyAsync(buf, comp_ptr(s, comp), sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+  } else {
+    if (!s->tmp_dev)
+      if (int rc = dalloc(s, &s->tmp_dev, sizeof(float) * n)) return rc;
+    const float inv = comp < 3 ? s->sc.ie[comp] : s->sc.ih[comp - 3];
+    launch_to_float(1, comp_ptr(s, comp), inv, s->tmp_dev, (long long)n, s->stream);
+    HIPCHK(hipMemcpyAsync(buf, s->tmp_dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int dtm_field_write(dtm_solver* s, int comp, const float* buf) {
+  if (!s || !buf || comp < 0 || comp > 5) return fail(DTM_ERR_ARG, "bad field write arguments");
+  const size_t n = (size_t)s->cfg.nx * s->cfg.ny * s->cfg.nz;
+  if (s->store == 0) {
+    HIPCHK(hipMemcpyAsync(comp_ptr(s, comp), buf, sizeof(float) * n, hipMemcpyHostToDevice, s->stream));
+  } else {
+    if (!s->tmp_dev)
+      if (int rc = dalloc(s, &s->tmp_dev, sizeof(float) * n)) return rc;
+    HIPCHK(hipMemcpyAsync(s->tmp_dev, buf, sizeof(float) * n, hipMemcpyHostToDevice, s->stream));
+    const float sc = comp < 3 ? s->sc.se[comp] : s->sc.sh[comp - 3];
+    launch_from_float(1, s->tmp_dev, sc, comp_ptr(s, comp), (long long)n, s->stream);
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int dtm_energy(dtm_solver* s, double* out) {
+  if (!s || !out) return fail(DTM_ERR_ARG, "null argument");
+  HIPCHK(hipMemsetAsync(s->energy_dev, 0, sizeof(double), s->stream));
+  launch_energy(s->store, s->g, fields_ptr(s), s->sc, s->mt, s->energy_dev, s->stream);
+  double v = 0;
+  HIPCHK(hipMemcpyAsync(&v, s->energy_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *out = v * s->cfg.dx * s->cfg.dy * s->cfg.dz;
+  return 0;
+}
+
+int dtm_time_steps(dtm_solver* s, int nsteps, double* ms) {
+  if (!s || !ms || nsteps < 0) return fail(DTM_ERR_ARG, "bad timing arguments");
+  if (int rc = start(s)) return rc;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, s->stream));
+  for (int t = 0; t < nsteps; t++)
+    if (int rc = one_step(s)) return rc;
+  HIPCHK(hipEventRecord(b, s->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float f = 0;
+  HIPCHK(hipEventElapsedTime(&f, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = f;
+  return 0;
+}
+
+const char* dtm_kernel_name(const dtm_solver* s) { return kernel_variant_name(s ? s->store : 0); }
+
+int dtm_set_kernel_timing(dtm_solver* s, int on) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  s->timing = on != 0;
+  return 0;
+}
+
+int dtm_kernel_times(dtm_solver* s, double* ms, long long* count) {
+  if (!s || !ms || !count) return fail(DTM_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (size_t q = 0; q < s->ev_used; q++) {
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, s->ev_pool[q][0], s->ev_pool[q][1]));
+    s->grp_ms[s->ev_group[q]] += f;
+    s->grp_cnt[s->ev_group[q]] += 1;
+  }
+  s->ev_used = 0;
+  for (int g = 0; g < 5; g++) {
+    ms[g] = s->grp_ms[g];
+    count[g] = s->grp_cnt[g];
+    s->grp_ms[g] = 0;
+    s->grp_cnt[g] = 0;
+  }
+  return 0;
+}
+
+long long dtm_group_cells(const dtm_solver* s, int g) {
+  if (!s) return -1;
+  auto cells = [](const Region& r) {
+    return (long long)std::max(0, r.i1 - r.i0) * std::max(0, r.j1 - r.j0) * std::max(0, r.k1 - r.k0);
+  };
+  if (g == 0 || g == 2) return cells(s->main);
+  if (g == 1 || g == 3) {
+    long long t = 0;
+    for (int q = 0; q < s->nedge; q++) t += cells(s->edge[q]);
+    return t;
+  }
+  return 0;
+}
+
+}  // extern "C"

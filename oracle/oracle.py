@@ -1,0 +1,186 @@
+"""ctypes binding of the oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+never by the product.  `Oracle(cfg, fp32=False)` drives liboracle64.so (the
+FP64 checker) or liboracle32.so (the FP32 OpenMP CPU baseline).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from dtmaxwell4_amd.abi import Config, Material, Tfsf  # noqa: F401  (struct layout)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+def build():
+    """Compile both oracle libraries with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(fp32=False):
+    name = "liboracle32.so" if fp32 else "liboracle64.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "orc_create": (P, [C.POINTER(Config)]),
+        "orc_destroy": (None, [P]),
+        "orc_set_materials": (C.c_int, [P, P, P, P]),
+        "orc_set_design_eps": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]),
+        "orc_add_probe": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "orc_set_dft": (C.c_int, [P, C.c_int, P]),
+        "orc_step": (C.c_int, [P, C.c_int]),
+        "orc_time_step": (C.c_longlong, [P]),
+        "orc_field_read": (C.c_int, [P, C.c_int, P]),
+        "orc_field_write": (C.c_int, [P, C.c_int, P]),
+        "orc_probe_read": (C.c_int, [P, P, C.c_longlong]),
+        "orc_dft_read": (C.c_int, [P, P]),
+        "orc_nthreads": (C.c_int, []),
+        "orc_real_size": (C.c_int, []),
+        "orc_incident": (C.c_double, [C.POINTER(Tfsf), C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double]),
+        "orc_wave": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "orc_pml_sigma": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, C.c_double]),
+        "orc_disp_coeffs": (None, [C.POINTER(Material), C.c_double, P, P, P]),
+        "orc_gyro_coeffs": (None, [C.POINTER(Material), C.c_double, P, P]),
+        "orc_set_eps": (None, [C.c_double] * 6 + [P]),
+        "orc_purcell": (C.c_double, [C.c_double] * 5),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype = r
+        f.argtypes = a
+    _libs[name] = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """CPU restatement of the stepping path, same calls as the HIP solver."""
+
+    def __init__(self, cfg, fp32=False):
+        self.L = lib(fp32)
+        self.cfg = cfg
+        self.shape = (cfg.nx, cfg.ny, cfg.nz)
+        self.h = self.L.orc_create(C.byref(cfg))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        self.nprobe = 0
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_materials(self, mx=None, my=None, mz=None):
+        arrs = []
+        for m in (mx, my, mz):
+            arrs.append(None if m is None else np.ascontiguousarray(m, dtype=np.uint8))
+        rc = self.L.orc_set_materials(self.h, *[None if a is None else _ptr(a) for a in arrs])
+        if rc < 0:
+            raise ValueError("bad material volume")
+
+    def set_design_eps(self, i0, j0, k0, eps):
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        ni, nj, nk = eps.shape
+        self.L.orc_set_design_eps(self.h, i0, j0, k0, ni, nj, nk, _ptr(eps))
+
+    def add_sensor(self, fld, x, y, z):
+        comp = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz").index(fld)
+        self.nprobe += 1
+        return self.L.orc_add_probe(self.h, 1, comp, x, y, z)
+
+    def add_probe(self, comp, i, j, k):
+        self.nprobe += 1
+        return self.L.orc_add_probe(self.h, 0, comp, i, j, k)
+
+    def set_dft(self, omegas):
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        self.nfreq = len(om)
+        self.L.orc_set_dft(self.h, len(om), _ptr(om))
+
+    def step(self, n):
+        self.L.orc_step(self.h, n)
+
+    @property
+    def time_step(self):
+        return self.L.orc_time_step(self.h)
+
+    def field(self, comp):
+        out = np.empty(self.shape, dtype=np.float64)
+        self.L.orc_field_read(self.h, comp, _ptr(out))
+        return out
+
+    def set_field(self, comp, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self.L.orc_field_write(self.h, comp, _ptr(a))
+
+    def fields(self):
+        return [self.field(c) for c in range(6)]
+
+    def probes(self):
+        rows = self.time_step
+        out = np.zeros((rows, max(self.nprobe, 1)), dtype=np.float64)
+        r = self.L.orc_probe_read(self.h, _ptr(out), rows)
+        return out[:r, :self.nprobe]
+
+    def dft(self):
+        out = np.zeros((self.nprobe, self.nfreq, 2), dtype=np.float64)
+        self.L.orc_dft_read(self.h, _ptr(out))
+        return out[..., 0] + 1j * out[..., 1]
+
+
+def incident(tfsf, comp, x, y, z, t):
+    return lib().orc_incident(C.byref(tfsf), comp, x, y, z, t)
+
+
+def wave(waveform, w, kEnv, t0, tw, t):
+    return lib().orc_wave(waveform, w, kEnv, t0, tw, t)
+
+
+def pml_sigma(h, n, plo, phi, d, m, R, vmax):
+    return lib().orc_pml_sigma(h, n, plo, phi, d, m, R, vmax)
+
+
+def disp_coeffs(mat, dt):
+    kE = np.zeros(3)
+    kEJ = np.zeros(8)
+    kJ = np.zeros(20)
+    lib().orc_disp_coeffs(C.byref(mat), dt, _ptr(kE), _ptr(kEJ), _ptr(kJ))
+    n = mat.npoles
+    return kE, kEJ[:2 * n].reshape(n, 2), kJ[:5 * n].reshape(n, 5)
+
+
+def gyro_coeffs(mat, dt):
+    A = np.zeros(9)
+    B = np.zeros(9)
+    lib().orc_gyro_coeffs(C.byref(mat), dt, _ptr(A), _ptr(B))
+    return A.reshape(3, 3), B.reshape(3, 3)
+
+
+def set_eps(eps, gx, gy, gz, w, dt):
+    out = np.zeros(18)
+    lib().orc_set_eps(eps, gx, gy, gz, w, dt, _ptr(out))
+    return out[:9].reshape(3, 3), out[9:].reshape(3, 3)
+
+
+def purcell(w, AE, phE, Ad, phd):
+    return lib().orc_purcell(w, AE, phE, Ad, phd)

@@ -1,0 +1,186 @@
+"""GPU parity: the HIP stepping path (through the C ABI) against the FP64
+oracle on the same seeded inputs.  Tolerance: relative L2 over all six field
+components <= 1e-5 for FP32 storage (the bound stated for the FP32 path);
+probe series and DFT bins to the same relative bound."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from dtmaxwell4_amd import abi, configs
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def solver(cfg):
+    from dtmaxwell4_amd.solver import Solver
+    lib = abi.load_library()
+    assert lib.dtm_device_count() > 0, "no HIP device visible"
+    return Solver(cfg)
+
+
+def run_pair(w, steps=None):
+    steps = w.steps if steps is None else steps
+    g = configs.apply(w, solver(w.cfg))
+    o = configs.apply(w, Oracle(w.cfg))
+    g.step(steps)
+    o.step(steps)
+    return g, o
+
+
+def check_fields(g, o, tol=TOL):
+    err = rel_l2(g.fields(), o.fields())
+    print(f"rel-L2 fields = {err:.3e}")
+    assert err <= tol, err
+    return err
+
+
+def test_periodic_eigenmode_matches_closed_form():
+    """An oblique plane-wave eigenmode of the periodic (2,4) lattice advances
+    by the closed-form discrete frequency (numerical dispersion relation)."""
+    nx, ny, nz = 32, 24, 16
+    d = 0.1
+    cfg = abi.default_config(nx, ny, nz, d, d, d, configs.dt_for(d, d, d, 0.6))
+    cfg.bc[:] = [abi.BC_PERIODIC] * 3
+    m = np.array([2, 1, 0])
+    k = 2 * np.pi * m / (np.array([nx, ny, nz]) * d)
+    K = (2 / d) * (9 / 8 * np.sin(k * d / 2) - 1 / 24 * np.sin(3 * k * d / 2))
+    w = 2 / cfg.dt * np.arcsin(cfg.dt * np.linalg.norm(K) / 2)
+    # E along z (K has no z part), H = -(K x E)/w~ components
+    x, y, z = configs.component_positions((nx, ny, nz), abi.EZ, d, d, d)
+    Ez = np.cos(k[0] * x + k[1] * y - 0 * z) * np.ones((nx, ny, nz))
+    wt = 2 / cfg.dt * np.sin(w * cfg.dt / 2)
+    g = solver(cfg)
+    g.set_field(abi.EZ, Ez)
+    # H = (1/w~) K x E at t=-dt/2 with E = e_z cos(phase): Hx = K_y/w~ ..., Hy = -K_x/w~
+    for comp, coef in ((abi.HX, K[1] / wt), (abi.HY, -K[0] / wt)):
+        x, y, z = configs.component_positions((nx, ny, nz), comp, d, d, d)
+        g.set_field(comp, coef * np.cos(k[0] * x + k[1] * y + w * cfg.dt / 2) * np.ones((nx, ny, nz)))
+    N = 150
+    g.step(N)
+    x, y, z = configs.component_positions((nx, ny, nz), abi.EZ, d, d, d)
+    ref = np.cos(k[0] * x + k[1] * y - w * N * cfg.dt) * np.ones((nx, ny, nz))
+    err = np.abs(g.field(abi.EZ) - ref).max()
+    print("eigenmode max err", err)
+    assert err < 5e-5
+
+
+def test_vacuum_dipole_small_cpml_sensors():
+    w = configs.vacuum_dipole(n=40, npml=8, steps=300)
+    g, o = run_pair(w)
+    check_fields(g, o)
+    gs, os_ = g.probes(), o.probes()
+    assert gs.shape == os_.shape == (300, 8)
+    assert rel_l2([gs], [os_]) <= TOL
+
+
+def test_vacuum_dipole_full_size_1000_steps():
+    """The 128^3 workload (10-cell CPML, soft Ez dipole, 8 seeded sensors,
+    1000 steps): FP32 HIP vs FP64 oracle."""
+    w = configs.vacuum_dipole()
+    g, o = run_pair(w)
+    err = check_fields(g, o)
+    serr = rel_l2([g.probes()], [o.probes()])
+    print(f"sensor rel-L2 {serr:.3e}")
+    assert serr <= TOL
+    assert err <= TOL
+
+
+def test_plane_wave_sphere_small():
+    w = configs.plane_wave_sphere(nx=64, ny=48, nz=48, npml=8, inset=4, radius=8, steps=300, periods_w=1.0, periods_t0=3.0)
+    g, o = run_pair(w)
+    check_fields(g, o)
+
+
+def test_plane_wave_no_scatterer_has_no_leakage():
+    """Without a scatterer the scattered-field region stays (numerically)
+    empty: TF/SF corrections cancel the incident wave outside the box."""
+    w = configs.plane_wave_sphere(nx=64, ny=40, nz=40, npml=8, inset=4, radius=0, eps=1.0, steps=200, periods_w=1.0, periods_t0=2.0)
+    g = configs.apply(w, solver(w.cfg))
+    g.step(200)
+    ez = g.field(abi.EZ)
+    b = 8 + 4
+    inside = np.abs(ez[b + 3:-b - 3, b + 3:-b - 3, b + 3:-b - 3]).max()
+    outside = np.abs(ez[:b - 3, :, :]).max()
+    print("inside", inside, "outside", outside)
+    assert inside > 0.1
+    assert outside < 2e-2 * inside
+
+
+def test_reference_dipole_tfsf_raised_cosine():
+    """The reference's wired source: DipoleTFSF, scattered field inside the
+    box, raised-cosine window, tStop gating at Ntime granularity."""
+    n = 40
+    d = 0.05
+    cfg = abi.default_config(n, n, n, d, d, d, configs.dt_for(d, d, d))
+    cfg.npml[:] = [6] * 6
+    cfg.ntime = 7
+    t = cfg.tfsf
+    t.kind, t.waveform, t.sf_inside = abi.TFSF_DIPOLE, abi.WAVE_RAISED_COS, 1
+    c = n * d / 2
+    t.box[:] = [c - 5 * d, c + 5 * d, c - 5 * d, c + 5 * d, c - 5 * d, c + 5 * d]
+    cL = 1.0
+    t.w = 2 * np.pi / 1.0
+    t.kEnv = 0.25
+    t.cSrc, t.Imp, t.amp = cL, cL, 1.0
+    t.tStop = 150 * cfg.dt
+    cfg.nmat = 2
+    cfg.mat[1].eps_inf = 3.0
+    mats = configs.sphere_mats((n, n, n), (d, d, d), (c + 10 * d, c, c), 4 * d, 1)
+    w = configs.Workload("dipole", cfg, mats=mats, sensors=[("Ex", 30, 20, 20), ("Ez", 20, 20, 30),
+                                                            ("Hy", 25, 21, 19), ("Hz", 20, 20, 0)], steps=250)
+    g, o = run_pair(w)
+    check_fields(g, o)
+    assert rel_l2([g.probes()], [o.probes()]) <= TOL
+
+
+def test_drude_lorentz_sphere_small():
+    w = configs.drude_lorentz_purcell(n=48, npml=8, radius=8, gap=3, nfreq=8, steps=400, cells_per_lambda=20)
+    g, o = run_pair(w)
+    check_fields(g, o)
+    gd, od = g.dft(), o.dft()
+    assert rel_l2([gd.real, gd.imag], [od.real, od.imag]) <= 10 * TOL
+
+
+def test_gyrotropic_slab_small():
+    w = configs.gyro_slab(nx=8, ny=8, nz=96, npml=10, thick=24, steps=400, cells_per_lambda=16)
+    g, o = run_pair(w)
+    check_fields(g, o)
+
+
+def test_design_eps_box():
+    w = configs.inverse_design(nx=48, ny=32, nz=32, npml=6, design=(12, 10, 8), steps=200, monitor=4)
+    g, o = run_pair(w)
+    check_fields(g, o)
+
+
+def test_zero_boundary_no_pml_and_lossy_material():
+    n = 24
+    d = 0.05
+    cfg = abi.default_config(n, n + 3, n - 2, d, d, d, configs.dt_for(d, d, d, 0.8))
+    cfg.nmat = 2
+    cfg.mat[1].eps_inf = 2.0
+    cfg.mat[1].sigma = 3.0
+    cfg.nsoft = 1
+    so = cfg.soft[0]
+    so.comp, so.i, so.j, so.k = abi.EY, 5, 6, 7
+    so.waveform, so.amp, so.w, so.tw, so.t0 = abi.WAVE_GAUSS_SINE, 1.0, 2 * np.pi, 0.3, 1.0
+    mats = configs.sphere_mats(cfg_shape := (cfg.nx, cfg.ny, cfg.nz), (d, d, d), (0.8, 0.8, 0.6), 0.25, 1)
+    w = configs.Workload("zero", cfg, mats=mats, steps=300)
+    g, o = run_pair(w)
+    check_fields(g, o)
+
+
+def test_fp16_storage_tracks_fp32():
+    w32 = configs.plane_wave_sphere(nx=64, ny=48, nz=48, npml=8, inset=4, radius=8, steps=300, periods_w=1.0, periods_t0=3.0)
+    w16 = configs.plane_wave_sphere(nx=64, ny=48, nz=48, npml=8, inset=4, radius=8, steps=300,
+                                    storage=abi.STORE_FP16, periods_w=1.0, periods_t0=3.0)
+    a = configs.apply(w32, solver(w32.cfg))
+    b = configs.apply(w16, solver(w16.cfg))
+    a.step(300)
+    b.step(300)
+    err = rel_l2(b.fields(), a.fields())
+    print("fp16 vs fp32 rel-L2", err)
+    assert err < 2e-2
