@@ -76,6 +76,7 @@ struct Ade {
   int gyro_on;
   float* Jg[3];
   const GyroMat* gmats;
+  float* curl[3];  // curl H of special cells, written by the E sweep
 };
 
 // TF/SF: integer inSF bounds on half-coordinates per axis (computed on the
@@ -85,8 +86,8 @@ struct Tfsf {
   int active;
   int kind, waveform, sf_inside;
   int hlo[3], hhi[3];  // inside <=> hlo <= h <= hhi on every axis
-  // shell test: cell in [clo,chi] on every axis and, on some axis a, in one
-  // of the face bands [fl0,fl1] or [fh0,fh1]
+  // cells that can see a face: [clo,chi] on every axis, face bands
+  // [fl0,fl1] and [fh0,fh1] along the face normal
   int clo[3], chi[3], fl0[3], fl1[3], fh0[3], fh1[3];
   double half_d[3];    // d_a / 2
   double w, kEnv, cSrc, Imp, t0, tw, amp;
@@ -113,6 +114,14 @@ struct Region {  // a box of cells [i0,i1) x [j0,j1) x [k0,k1)
   int i0, i1, j0, j1, k0, k1;
 };
 
+// TF/SF face bands: for each axis a, up to two boxes of cells whose stencil
+// along a can cross a face of the box (the other axes span the box).
+struct TfsfBands {
+  int n;
+  Region r[6];
+  int axis[6];
+};
+
 struct ProbeDesc {
   int kind, comp, i, j, k;
 };
@@ -137,13 +146,20 @@ struct Soft {
 };
 
 // Launchers (dtm_kernels.hip).  All enqueue on `st`.
-// Which region groups a launch covers: bit 0 interior, bit 1 edge slabs.
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Tfsf& tf, double t_inc, const Region* regs, int nreg, const Region& main,
-                     hipStream_t st);
-void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Materials& mt, const Ade& ade, const Tfsf& tf, double t_inc,
                      const Region* regs, int nreg, const Region& main, hipStream_t st);
+void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, const Region& main,
+                     hipStream_t st);
+// TF/SF corrections applied after a sweep (half = 0: H, 1: E).  The update is
+// linear in the stencil sums, so the correction is additive, including its
+// CPML psi and ADE J terms.
+void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                 const Materials& mt, const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double t_inc,
+                 hipStream_t st);
+// Dispersive / gyrotropic cells over the ADE bounding box (after the E sweep).
+void launch_ade(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt,
+                const Ade& ade, hipStream_t st);
 void launch_soft(int store, const Grid& g, const void* fields, const Scales& sc, const Soft& so,
                  hipStream_t st);
 void launch_record(int store, const Grid& g, const void* fields, const Scales& sc, const Probes& pr,

@@ -20,6 +20,8 @@
 #include <math.h>
 #include <stdio.h>
 
+#include <algorithm>
+
 #include "dtm_internal.h"
 
 namespace dtm {
@@ -214,14 +216,6 @@ __device__ __noinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, 
   return (float)acc;
 }
 
-__device__ __forceinline__ bool near_face(const Tfsf& s, int a, int v) {
-  return (v >= s.fl0[a] && v <= s.fl1[a]) || (v >= s.fh0[a] && v <= s.fh1[a]);
-}
-__device__ __forceinline__ bool in_shell(const Tfsf& s, int i, int j, int k) {
-  if (i < s.clo[0] || i > s.chi[0] || j < s.clo[1] || j > s.chi[1] || k < s.clo[2] || k > s.chi[2]) return false;
-  return near_face(s, 0, i) || near_face(s, 1, j) || near_face(s, 2, k);
-}
-
 // -------------------------------------------------------- CPML helpers
 __device__ __forceinline__ int slab_index(int v, int n, int plo, int phi) {
   if (v < plo) return v;
@@ -246,8 +240,7 @@ __device__ __forceinline__ void cpml_apply(float* psi, long long pq, float b, fl
 
 // ================================================================ H sweep
 template <int STORE, bool GEN>
-__global__ __launch_bounds__(256) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Tfsf tf,
-                                                  double tinc, RegionSet rs) {
+__global__ __launch_bounds__(256) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, RegionSet rs) {
   int i, j, k;
   if (!locate(rs, i, j, k)) return;
   const long long sx = g.sx, sy = g.sy;
@@ -284,15 +277,6 @@ __global__ __launch_bounds__(256) void k_update_h(Grid g, Fields<STORE> f, Scale
                              ldn(Ex, rowq + kp1, kp1 >= 0, iex));
   float dyEx = g.rd[1] * d24(ldn(Ex, ym2, oym2, iex), ldn(Ex, ym1, oym1, iex), ex0, ldn(Ex, yp1, oyp1, iex));
 
-  if (tf.active && in_shell(tf, i, j, k)) {
-    // Hx (2i+1,2j,2k), Hy (2i,2j+1,2k), Hz (2i,2j,2k+1)
-    dyEz += g.rd[1] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k, 1, tinc);
-    dzEy += g.rd[2] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k, 2, tinc);
-    dzEx += g.rd[2] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k, 2, tinc);
-    dxEz += g.rd[0] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k, 0, tinc);
-    dxEy += g.rd[0] * tfsf_dcorr(tf, 2 * i, 2 * j, 2 * k + 1, 0, tinc);
-    dyEx += g.rd[1] * tfsf_dcorr(tf, 2 * i, 2 * j, 2 * k + 1, 1, tinc);
-  }
   if (pm.on) {
     // z axis (all regions): slot 0 = comp x (Hx: D_z Ey), slot 1 = comp y (Hy: D_z Ex)
     const int sz = slab_index(k, g.nz, pm.plo[2], pm.phi[2]);
@@ -328,49 +312,138 @@ __global__ __launch_bounds__(256) void k_update_h(Grid g, Fields<STORE> f, Scale
   st(Hz, q, ld(Hz, q, sc.ih[2]) - dt * (dxEy - dyEx), sc.sh[2]);
 }
 
-// ======================================================== E sweep helpers
+// ============================================================ ADE cells
 // Dispersive / gyrotropic cells: countDisp (dispersion.cpp:85-99) and the
-// magnetised-plasma current.  curl[] is curl H (with CPML / TF/SF applied).
-template <typename T>
-__device__ __noinline__ void e_special(const Grid& g, T* E0, T* E1, T* E2, const Scales& sc, const Ade& ade,
-                                       const ClassEntry& ce, const float inv_eps_override[3], bool over,
-                                       long long q, int i, int j, int k, float c0, float c1, float c2) {
-  T* E[3] = {E0, E1, E2};
-  float curl[3] = {c0, c1, c2};
-  const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
-  float En[3];
-  for (int c = 0; c < 3; c++) En[c] = ld(E[c], q, sc.ie[c]);
-  if ((ce.special & 8) && ade.gyro_on) {
-    const GyroMat& gm = ade.gmats[ce.mat[0]];
-    float J0[3] = {ade.Jg[0][b], ade.Jg[1][b], ade.Jg[2][b]};
-    for (int r = 0; r < 3; r++) {
-      float v = 0.f;
-      for (int t = 0; t < 3; t++) v += gm.A[3 * r + t] * J0[t] + gm.B[3 * r + t] * En[t];
-      ade.Jg[r][b] = v;
-      curl[r] -= v;
+// magnetised-plasma current, from the curl H the E sweep stored.
+template <int STORE>
+__global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Scales sc, Materials mt, Ade ade) {
+  const long long nb = (long long)ade.ni * ade.nj * ade.nk;
+  for (long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x; b < nb; b += (long long)gridDim.x * blockDim.x) {
+    const int k = ade.k0 + (int)(b % ade.nk);
+    const int j = ade.j0 + (int)((b / ade.nk) % ade.nj);
+    const int i = ade.i0 + (int)(b / ((long long)ade.nk * ade.nj));
+    const long long q = i * g.sx + j * g.sy + k;
+    const ClassEntry ce = mt.table[mt.cls ? mt.cls[q] : 0];
+    if (!ce.special) continue;
+    if (mt.d_on) {
+      const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
+      if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) continue;
+    }
+    float curl[3] = {ade.curl[0][b], ade.curl[1][b], ade.curl[2][b]};
+    float En[3];
+    for (int c = 0; c < 3; c++) En[c] = ld(f.E[c], q, sc.ie[c]);
+    if ((ce.special & 8) && ade.gyro_on) {
+      const GyroMat& gm = ade.gmats[ce.mat[0]];
+      const float J0[3] = {ade.Jg[0][b], ade.Jg[1][b], ade.Jg[2][b]};
+      for (int r = 0; r < 3; r++) {
+        float v = 0.f;
+        for (int t = 0; t < 3; t++) v += gm.A[3 * r + t] * J0[t] + gm.B[3 * r + t] * En[t];
+        ade.Jg[r][b] = v;
+        curl[r] -= v;
+      }
+    }
+    for (int c = 0; c < 3; c++) {
+      if (ce.special & (1 << c)) {
+        const AdeMat& am = ade.mats[ce.mat[c]];
+        const float Ep = En[c];
+        const float Em = ade.Em[c][b];
+        const float rotdt = g.dt * curl[c];
+        float sum = 0.f;
+        for (int p = 0; p < am.np; p++) sum += am.kEJ[p][0] * ade.Jp[c][p][b] + am.kEJ[p][1] * ade.Jm[c][p][b];
+        const float Enew = am.kE[0] * Em + am.kE[1] * Ep + am.kE[2] * rotdt - sum;
+        for (int p = 0; p < am.np; p++) {
+          const float jp = ade.Jp[c][p][b], jm = ade.Jm[c][p][b];
+          const float jn = am.kJ[p][0] * jp + am.kJ[p][1] * jm + am.kJ[p][2] * Enew + am.kJ[p][3] * Em +
+                           am.kJ[p][4] * Ep;
+          ade.Jm[c][p][b] = jp;
+          ade.Jp[c][p][b] = jn;
+        }
+        ade.Em[c][b] = Ep;
+        st(f.E[c], q, Enew, sc.se[c]);
+      } else {
+        st(f.E[c], q, En[c] + g.dt * ce.inv_eps[c] * curl[c], sc.se[c]);
+      }
     }
   }
-  for (int c = 0; c < 3; c++) {
-    if (!over && (ce.special & (1 << c))) {
-      const AdeMat& am = ade.mats[ce.mat[c]];
-      const float Ep = En[c];
-      const float Em = ade.Em[c][b];
-      const float rotdt = g.dt * curl[c];
-      float sum = 0.f;
-      for (int p = 0; p < am.np; p++) sum += am.kEJ[p][0] * ade.Jp[c][p][b] + am.kEJ[p][1] * ade.Jm[c][p][b];
-      const float Enew = am.kE[0] * Em + am.kE[1] * Ep + am.kE[2] * rotdt - sum;
-      for (int p = 0; p < am.np; p++) {
-        const float jp = ade.Jp[c][p][b], jm = ade.Jm[c][p][b];
-        const float jn = am.kJ[p][0] * jp + am.kJ[p][1] * jm + am.kJ[p][2] * Enew + am.kJ[p][3] * Em +
-                         am.kJ[p][4] * Ep;
-        ade.Jm[c][p][b] = jp;
-        ade.Jp[c][p][b] = jn;
+}
+
+// ====================================================== TF/SF correction
+// For a cell in the face band of axis a, the two components with a
+// derivative along a get the TF/SF term of genAlg.py:219-221 added to their
+// stencil sum.  The sweep is linear in that sum, so the correction is
+//   H_c -= dt * s * (1 + c_pml) * D,   psi += c_pml * D      (H half)
+//   E_c += dt * ie * s * (1 + c_pml) * D, psi += c_pml * D   (E half)
+// with s = +1 for the first curl term and -1 for the second; a dispersive
+// component gets C3 * s * (1+c_pml) * D and its J^{n+1} the matching
+// kJ2 * dE (dispersion.cpp:84-99 is linear in the curl).
+template <int STORE, int HALF>
+__global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt, Ade ade,
+                                              Tfsf tf, TfsfBands bands, double tinc) {
+  long long acc = 0;
+  const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  int r = 0;
+  long long loc = gid;
+  for (; r < bands.n; r++) {
+    const Region& R = bands.r[r];
+    const long long cnt = (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
+    if (loc < cnt) break;
+    loc -= cnt;
+    acc += cnt;
+  }
+  if (r >= bands.n) return;
+  const Region& R = bands.r[r];
+  const int a = bands.axis[r];
+  const int nk = R.k1 - R.k0, nj = R.j1 - R.j0;
+  const int k = R.k0 + (int)(loc % nk);
+  const int j = R.j0 + (int)((loc / nk) % nj);
+  const int i = R.i0 + (int)(loc / ((long long)nk * nj));
+  const long long q = i * g.sx + j * g.sy + k;
+  const int vpos[3] = {i, j, k};
+  // components with a derivative along a: c1 = (a+2)%3 (first curl term),
+  // c2 = (a+1)%3 (second curl term); psi slots 1 and 0 of axis a.
+  static constexpr int off[6][3] = {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int t = 0; t < 2; t++) {
+    const int c = t == 0 ? (a + 2) % 3 : (a + 1) % 3;
+    const float sgn = t == 0 ? 1.0f : -1.0f;
+    const int slot = t == 0 ? 1 : 0;
+    const int comp = HALF == 0 ? 3 + c : c;
+    const int hx = 2 * i + off[comp][0], hy = 2 * j + off[comp][1], hz = 2 * k + off[comp][2];
+    const float D = g.rd[a] * tfsf_dcorr(tf, hx, hy, hz, a, tinc);
+    if (D == 0.0f) continue;
+    float cp = 0.0f;
+    if (pm.on) {
+      const int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+      const int si = slab_index(vpos[a], n, pm.plo[a], pm.phi[a]);
+      if (si >= 0) {
+        cp = HALF == 0 ? pm.cH[a][vpos[a]] : pm.cE[a][vpos[a]];
+        float* psi = HALF == 0 ? pm.psiH[a][slot] : pm.psiE[a][slot];
+        const long long pq = psi_index(g, pm, a, si, i, j, k);
+        psi[pq] += cp * D;
       }
-      ade.Em[c][b] = Ep;
-      st(E[c], q, Enew, sc.se[c]);
+    }
+    const float dcurl = sgn * (1.0f + cp) * D;
+    if (HALF == 0) {
+      st(f.H[c], q, ld(f.H[c], q, sc.ih[c]) - g.dt * dcurl, sc.sh[c]);
+      continue;
+    }
+    const ClassEntry ce = mt.table[mt.cls ? mt.cls[q] : 0];
+    float ie = ce.inv_eps[c];
+    bool over = false;
+    if (mt.d_on) {
+      const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
+      if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) {
+        ie = mt.d_inv_eps[((long long)di * mt.d_nj + dj) * mt.d_nk + dk];
+        over = true;
+      }
+    }
+    if (!over && (ce.special & (1 << c)) && ade.on) {
+      const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
+      const AdeMat& am = ade.mats[ce.mat[c]];
+      const float dE = am.kE[2] * g.dt * dcurl;
+      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + dE, sc.se[c]);
+      for (int p = 0; p < am.np; p++) ade.Jp[c][p][b] += am.kJ[p][2] * dE;
     } else {
-      const float ie = over ? inv_eps_override[c] : ce.inv_eps[c];
-      st(E[c], q, En[c] + g.dt * ie * curl[c], sc.se[c]);
+      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + g.dt * ie * dcurl, sc.se[c]);
     }
   }
 }
@@ -378,7 +451,7 @@ __device__ __noinline__ void e_special(const Grid& g, T* E0, T* E1, T* E2, const
 // ================================================================ E sweep
 template <int STORE, bool GEN>
 __global__ __launch_bounds__(256) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
-                                                  Ade ade, Tfsf tf, double tinc, RegionSet rs) {
+                                                  Ade ade, RegionSet rs) {
   __shared__ ClassEntry tbl[kMaxClasses];
   const int tid = threadIdx.x + threadIdx.y * BZ;
   for (int t = tid; t < mt.nclass; t += BZ * BY) tbl[t] = mt.table[t];
@@ -418,15 +491,6 @@ __global__ __launch_bounds__(256) void k_update_e(Grid g, Fields<STORE> f, Scale
                              ldn(Hx, rowq + kp2, kp2 >= 0, ihx));
   float dyHx = g.rd[1] * d24(ldn(Hx, ym1, oym1, ihx), hx0, ldn(Hx, yp1, oyp1, ihx), ldn(Hx, yp2, oyp2, ihx));
 
-  if (tf.active && in_shell(tf, i, j, k)) {
-    // Ex (2i,2j+1,2k+1), Ey (2i+1,2j,2k+1), Ez (2i+1,2j+1,2k)
-    dyHz += g.rd[1] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k + 1, 1, tinc);
-    dzHy += g.rd[2] * tfsf_dcorr(tf, 2 * i, 2 * j + 1, 2 * k + 1, 2, tinc);
-    dzHx += g.rd[2] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k + 1, 2, tinc);
-    dxHz += g.rd[0] * tfsf_dcorr(tf, 2 * i + 1, 2 * j, 2 * k + 1, 0, tinc);
-    dxHy += g.rd[0] * tfsf_dcorr(tf, 2 * i + 1, 2 * j + 1, 2 * k, 0, tinc);
-    dyHx += g.rd[1] * tfsf_dcorr(tf, 2 * i + 1, 2 * j + 1, 2 * k, 1, tinc);
-  }
   if (pm.on) {
     const int sz = slab_index(k, g.nz, pm.plo[2], pm.phi[2]);
     if (sz >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
@@ -467,8 +531,11 @@ __global__ __launch_bounds__(256) void k_update_e(Grid g, Fields<STORE> f, Scale
   auto* Ex = f.E[0];
   auto* Ey = f.E[1];
   auto* Ez = f.E[2];
-  if (ce.special) {
-    e_special(g, Ex, Ey, Ez, sc, ade, ce, ie, over, q, i, j, k, cx, cy, cz);
+  if (ce.special && !over) {  // finished by k_ade
+    const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
+    ade.curl[0][b] = cx;
+    ade.curl[1][b] = cy;
+    ade.curl[2][b] = cz;
     return;
   }
   const float dt = g.dt;
@@ -573,53 +640,98 @@ static void check_launch(const char* what) {
 }
 
 template <int STORE>
-static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Tfsf& tf, double tinc,
-                  const Region* regs, int nreg, const Region& main, hipStream_t st) {
+static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
+                  const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, BY, 1);
   RegionSet rm = make_set(&main, main.i1 > main.i0 ? 1 : 0);
   if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, tf, tinc, rm);
+    hipLaunchKernelGGL((k_update_h<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, rm);
     check_launch("k_update_h<main>");
   }
   RegionSet rg = make_set(regs, nreg);
   if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, tf, tinc, rg);
+    hipLaunchKernelGGL((k_update_h<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
     check_launch("k_update_h<edge>");
   }
 }
 
 template <int STORE>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
-                  const Ade& ade, const Tfsf& tf, double tinc, const Region* regs, int nreg, const Region& main,
-                  hipStream_t st) {
+                  const Ade& ade, const Region* regs, int nreg, const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, BY, 1);
   RegionSet rm = make_set(&main, main.i1 > main.i0 ? 1 : 0);
   if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, tf, tinc,
-                       rm);
+    hipLaunchKernelGGL((k_update_e<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rm);
     check_launch("k_update_e<main>");
   }
   RegionSet rg = make_set(regs, nreg);
   if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, tf, tinc,
-                       rg);
+    hipLaunchKernelGGL((k_update_e<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
     check_launch("k_update_e<edge>");
   }
 }
 
-void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Tfsf& tf,
-                     double t_inc, const Region* regs, int nreg, const Region& main, hipStream_t st) {
-  if (store == 0) upd_h<0>(g, fields, sc, pm, tf, t_inc, regs, nreg, main, st);
-  else upd_h<1>(g, fields, sc, pm, tf, t_inc, regs, nreg, main, st);
+void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                     const Region* regs, int nreg, const Region& main, hipStream_t st) {
+  if (store == 0) upd_h<0>(g, fields, sc, pm, regs, nreg, main, st);
+  else upd_h<1>(g, fields, sc, pm, regs, nreg, main, st);
 }
 
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Materials& mt, const Ade& ade, const Tfsf& tf, double t_inc, const Region* regs,
-                     int nreg, const Region& main, hipStream_t st) {
-  if (store == 0) upd_e<0>(g, fields, sc, pm, mt, ade, tf, t_inc, regs, nreg, main, st);
-  else upd_e<1>(g, fields, sc, pm, mt, ade, tf, t_inc, regs, nreg, main, st);
+                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, const Region& main,
+                     hipStream_t st) {
+  if (store == 0) upd_e<0>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+  else upd_e<1>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+}
+
+// One launch per face-normal axis: bands of different axes overlap at the
+// box edges and both update the shared component there (e.g. Hz from the x
+// and y faces), so they must not run concurrently.
+template <int STORE, int HALF>
+static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
+                    const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double tinc, hipStream_t st) {
+  const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
+  for (int a = 0; a < 3; a++) {
+    TfsfBands sub{};
+    long long n = 0;
+    for (int r = 0; r < bands.n; r++) {
+      if (bands.axis[r] != a) continue;
+      const Region& R = bands.r[r];
+      sub.r[sub.n] = R;
+      sub.axis[sub.n] = a;
+      sub.n++;
+      n += (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
+    }
+    if (n <= 0) continue;
+    hipLaunchKernelGGL((k_tfsf<STORE, HALF>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, f, sc, pm, mt,
+                       ade, tf, sub, tinc);
+    check_launch("k_tfsf");
+  }
+}
+
+void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                 const Materials& mt, const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double t_inc,
+                 hipStream_t st) {
+  if (store == 0 && half == 0) tfsf_go<0, 0>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
+  if (store == 0 && half == 1) tfsf_go<0, 1>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
+  if (store == 1 && half == 0) tfsf_go<1, 0>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
+  if (store == 1 && half == 1) tfsf_go<1, 1>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
+}
+
+void launch_ade(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt, const Ade& ade,
+                hipStream_t st) {
+  if (!ade.on) return;
+  const long long nb = (long long)ade.ni * ade.nj * ade.nk;
+  const unsigned blocks = (unsigned)std::min<long long>((nb + 255) / 256, 65536);
+  if (store == 0)
+    hipLaunchKernelGGL(k_ade<0>, dim3(blocks), dim3(256), 0, st, g, *static_cast<const Fields<0>*>(fields), sc, mt,
+                       ade);
+  else
+    hipLaunchKernelGGL(k_ade<1>, dim3(blocks), dim3(256), 0, st, g, *static_cast<const Fields<1>*>(fields), sc, mt,
+                       ade);
+  check_launch("k_ade");
 }
 
 void launch_soft(int store, const Grid&, const void* fields, const Scales& sc, const Soft& so, hipStream_t st) {

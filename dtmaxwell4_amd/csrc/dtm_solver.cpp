@@ -68,6 +68,7 @@ struct dtm_solver {
   std::vector<float> design_host;
   // TF/SF
   Tfsf tf{};
+  TfsfBands bands{};
   // probes
   std::vector<ProbeDesc> probes;
   ProbeDesc* probes_dev = nullptr;
@@ -338,6 +339,37 @@ static int setup_tfsf(dtm_solver* s) {
     t.pos[a] = c.tfsf.pos[a];
     t.dip[a] = 0.5 * (c.tfsf.box[2 * a] + c.tfsf.box[2 * a + 1]);
   }
+  // face bands, clamped to the grid; lo/hi merged when they meet
+  TfsfBands& B = s->bands;
+  B.n = 0;
+  auto clampr = [](int lo, int hi, int n, int& o0, int& o1) {
+    o0 = std::max(lo, 0);
+    o1 = std::min(hi + 1, n);
+    return o1 > o0;
+  };
+  for (int a = 0; a < 3; a++) {
+    if (t.hlo[a] > t.hhi[a]) continue;
+    int ext0[3], ext1[3];
+    bool ok = true;
+    for (int b = 0; b < 3; b++) ok = ok && clampr(t.clo[b], t.chi[b], n[b], ext0[b], ext1[b]);
+    if (!ok) continue;
+    int bands[2][2];
+    int nb = 0;
+    if (t.fh0[a] <= t.fl1[a] + 1) {
+      if (clampr(t.fl0[a], t.fh1[a], n[a], bands[0][0], bands[0][1])) nb = 1;
+    } else {
+      if (clampr(t.fl0[a], t.fl1[a], n[a], bands[nb][0], bands[nb][1])) nb++;
+      if (clampr(t.fh0[a], t.fh1[a], n[a], bands[nb][0], bands[nb][1])) nb++;
+    }
+    for (int q = 0; q < nb; q++) {
+      int r0[3] = {ext0[0], ext0[1], ext0[2]}, r1[3] = {ext1[0], ext1[1], ext1[2]};
+      r0[a] = bands[q][0];
+      r1[a] = bands[q][1];
+      B.r[B.n] = Region{r0[0], r1[0], r0[1], r1[1], r0[2], r1[2]};
+      B.axis[B.n] = a;
+      B.n++;
+    }
+  }
   return 0;
 }
 
@@ -509,6 +541,7 @@ static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
       if (c.mat[mm].gyro) gyro_coeffs(c.mat[mm], c.dt, gm[mm]);
     }
     for (int q = 0; q < 3; q++) {
+      if (int rc = dalloc(s, &A.curl[q], sizeof(float) * nb)) return rc;
       if (int rc = dalloc(s, &A.Em[q], sizeof(float) * nb)) return rc;
       for (int p = 0; p < maxp; p++) {
         if (int rc = dalloc(s, &A.Jp[q][p], sizeof(float) * nb)) return rc;
@@ -773,19 +806,26 @@ static int one_step(dtm_solver* s) {
   const double th = (double)n * c.dt, te = ((double)n + 0.5) * c.dt;
   // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
   ev_begin(s, 0);
-  launch_update_h(s->store, s->g, F, s->sc, s->pm, s->tf, th, nullptr, 0, s->main, s->stream);
+  launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main, s->stream);
   ev_end(s);
   ev_begin(s, 1);
-  launch_update_h(s->store, s->g, F, s->sc, s->pm, s->tf, th, s->edge, s->nedge, kEmpty, s->stream);
+  launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge, s->nedge, kEmpty, s->stream);
   ev_end(s);
+  if (s->tf.active) {
+    ev_begin(s, 4);
+    launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, th, s->stream);
+    ev_end(s);
+  }
   // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
   ev_begin(s, 2);
-  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, te, nullptr, 0, s->main, s->stream);
+  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main, s->stream);
   ev_end(s);
   ev_begin(s, 3);
-  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, te, s->edge, s->nedge, kEmpty, s->stream);
+  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge, s->nedge, kEmpty, s->stream);
   ev_end(s);
   ev_begin(s, 4);
+  launch_ade(s->store, s->g, F, s->sc, s->mt, s->ade, s->stream);
+  if (s->tf.active) launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, te, s->stream);
   if (c.nsoft > 0) {
     Soft so{};
     so.n = c.nsoft;
