@@ -30,56 +30,6 @@ constexpr int BZ = 64;  // z extent of a block (one wavefront)
 constexpr int BY = 4;   // y extent of a block
 constexpr int NXCD = 8;
 
-struct RegionSet {
-  int n;
-  Region r[6];
-  int nbk[6], nbj[6];
-  int start[7];
-  int total;  // logical blocks
-  int per;    // logical blocks per XCD
-};
-
-static RegionSet make_set(const Region* regs, int nreg) {
-  RegionSet s{};
-  int acc = 0;
-  int m = 0;
-  for (int t = 0; t < nreg && m < 6; t++) {
-    const Region& r = regs[t];
-    if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
-    s.r[m] = r;
-    s.nbk[m] = (r.k1 - r.k0 + BZ - 1) / BZ;
-    s.nbj[m] = (r.j1 - r.j0 + BY - 1) / BY;
-    s.start[m] = acc;
-    acc += s.nbk[m] * s.nbj[m] * (r.i1 - r.i0);
-    m++;
-  }
-  s.n = m;
-  s.start[m] = acc;
-  s.total = acc;
-  s.per = (acc + NXCD - 1) / NXCD;
-  return s;
-}
-
-// Maps the hardware block id to (region, cell).  Returns false for padding
-// blocks and for threads outside their region.
-__device__ __forceinline__ bool locate(const RegionSet& rs, int& i, int& j, int& k) {
-  const int b = blockIdx.x;
-  const int L = (b % NXCD) * rs.per + b / NXCD;  // XCD-contiguous logical id
-  if (L >= rs.total) return false;
-  int r = 0;
-  while (r + 1 < rs.n && L >= rs.start[r + 1]) r++;
-  int loc = L - rs.start[r];
-  const int bk = loc % rs.nbk[r];
-  loc /= rs.nbk[r];
-  const int bj = loc % rs.nbj[r];
-  const int bi = loc / rs.nbj[r];
-  const Region& R = rs.r[r];
-  k = R.k0 + bk * BZ + (int)threadIdx.x;
-  j = R.j0 + bj * BY + (int)threadIdx.y;
-  i = R.i0 + bi;
-  return k < R.k1 && j < R.j1 && i < R.i1;
-}
-
 // ------------------------------------------------------- storage helpers
 __device__ __forceinline__ float ld(const float* F, long long q, float) { return F[q]; }
 __device__ __forceinline__ float ld(const __half* F, long long q, float inv) {
@@ -109,7 +59,7 @@ __device__ __forceinline__ float d24(float f0, float f1, float f2, float f3) {
 }
 
 // ------------------------------------------------------- incident fields
-__device__ void wave3(int waveform, double p, double w, double kEnv, double t0, double tw,
+__device__ __forceinline__ void wave3(int waveform, double p, double w, double kEnv, double t0, double tw,
                       double& f0, double& f1, double& f2) {
   if (waveform == 0) {  // raised-cosine window (signal.hpp:71-77)
     const double kp = p * kEnv;
@@ -139,7 +89,7 @@ __device__ void wave3(int waveform, double p, double w, double kEnv, double t0, 
 
 // Incident value of component comp (0..5) at half-coordinates h, time t.
 // Dipole: signal.hpp:58-96; plane wave: signal.hpp:17-49.
-__device__ __noinline__ double incident(const Tfsf& s, int comp, int hx, int hy, int hz, double t) {
+__device__ __forceinline__ double incident(const Tfsf& s, int comp, int hx, int hy, int hz, double t) {
   const double x = hx * s.half_d[0], y = hy * s.half_d[1], z = hz * s.half_d[2];
   if (s.kind == 1) {
     double n0 = x - s.dip[0], n1 = y - s.dip[1], n2 = z - s.dip[2];
@@ -198,7 +148,7 @@ __device__ __forceinline__ int comp_at(int hx, int hy, int hz) {
 
 // Sum over the 4 stencil neighbours along axis a of component-at-(hp) of the
 // TF/SF corrections (genAlg.py:219-221) weighted by (+1,-27,+27,-1).
-__device__ __noinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t) {
+__device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t) {
   const bool psf = in_sf(s, hx, hy, hz);
   const int ho[4] = {-3, -1, 1, 3};
   const double wt[4] = {1.0, -27.0, 27.0, -1.0};
@@ -238,78 +188,202 @@ __device__ __forceinline__ void cpml_apply(float* psi, long long pq, float b, fl
   D += v;
 }
 
+// ===================================================== marching sweeps
+// A block owns a tile of 64 (z) x MBY (y) columns and marches along x over
+// a segment of planes.  Per thread, the x-neighbours of its column live in a
+// 4-deep register queue (the tilted space-time march of the reference's
+// torre, genAlg.py:420-499, with the rag shuffles of copyrags.cuh replaced
+// by a register queue); the y/z neighbours of the current plane come from a
+// double-buffered LDS tile with a 3-cell halo.  One barrier per plane.
+constexpr int MBY = 8;    // y rows per block
+constexpr int MLZ = 67;   // LDS row length: 64 + 3 halo columns
+
+struct MarchSet {
+  int n;
+  Region r[6];
+  int nbk[6], nbj[6], nbs[6];
+  int start[7];
+  int total, per, seg;
+};
+
+static MarchSet make_march(const Region* regs, int nreg, int seg) {
+  MarchSet s{};
+  s.seg = seg;
+  int acc = 0, m = 0;
+  for (int t = 0; t < nreg && m < 6; t++) {
+    const Region& r = regs[t];
+    if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
+    s.r[m] = r;
+    s.nbk[m] = (r.k1 - r.k0 + BZ - 1) / BZ;
+    s.nbj[m] = (r.j1 - r.j0 + MBY - 1) / MBY;
+    s.nbs[m] = (r.i1 - r.i0 + seg - 1) / seg;
+    s.start[m] = acc;
+    acc += s.nbk[m] * s.nbj[m] * s.nbs[m];
+    m++;
+  }
+  s.n = m;
+  s.start[m] = acc;
+  s.total = acc;
+  s.per = (acc + NXCD - 1) / NXCD;
+  return s;
+}
+
+struct MarchTile {
+  int r, i0, i1, j0, k0;
+};
+
+__device__ __forceinline__ bool march_locate(const MarchSet& ms, MarchTile& t) {
+  const int b = blockIdx.x;
+  const int L = (b % NXCD) * ms.per + b / NXCD;  // XCD-contiguous logical id
+  if (L >= ms.total) return false;
+  int r = 0;
+  while (r + 1 < ms.n && L >= ms.start[r + 1]) r++;
+  int loc = L - ms.start[r];
+  const int bk = loc % ms.nbk[r];
+  loc /= ms.nbk[r];
+  const int bj = loc % ms.nbj[r];
+  const int bs = loc / ms.nbj[r];
+  const Region& R = ms.r[r];
+  t.r = r;
+  t.k0 = R.k0 + bk * BZ;
+  t.j0 = R.j0 + bj * MBY;
+  t.i0 = R.i0 + bs * ms.seg;
+  t.i1 = min(t.i0 + ms.seg, R.i1);
+  return true;
+}
+
+// In-plane offset (j*sy + k) of a position under the boundary rule, or -1.
+__device__ __forceinline__ long long plane_off(const Grid& g, int j, int k) {
+  const int jj = nbr(j, 0, g.ny, g.per[1]), kk = nbr(k, 0, g.nz, g.per[2]);
+  return (jj < 0 || kk < 0) ? -1LL : (long long)jj * g.sy + kk;
+}
+template <bool GEN>
+__device__ __forceinline__ long long plane_base(const Grid& g, int i) {
+  if (!GEN) return (long long)i * g.sx;
+  const int ii = nbr(i, 0, g.nx, g.per[0]);
+  return ii < 0 ? -1LL : (long long)ii * g.sx;
+}
+template <typename T>
+__device__ __forceinline__ float ldp(const T* F, long long base, long long off, float inv) {
+  return (base < 0 || off < 0) ? 0.0f : ld(F, base + off, inv);
+}
+
+// Barrier that waits only for this wave's LDS traffic, so global loads
+// issued for the next plane stay in flight across it (__syncthreads would
+// also drain every outstanding global load).  The "memory" clobber keeps the
+// compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // ================================================================ H sweep
 template <int STORE, bool GEN>
-__global__ __launch_bounds__(256) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, RegionSet rs) {
-  int i, j, k;
-  if (!locate(rs, i, j, k)) return;
-  const long long sx = g.sx, sy = g.sy;
-  const long long q = i * sx + j * sy + k;
-  // z neighbours (offsets -2, -1, +1)
-  const int km2 = nbr(k, -2, g.nz, g.per[2]), km1 = nbr(k, -1, g.nz, g.per[2]),
-            kp1 = nbr(k, 1, g.nz, g.per[2]);
-  const long long rowq = q - k;
-  long long xm2, xm1, xp1, ym2, ym1, yp1;
-  bool oxm2 = true, oxm1 = true, oxp1 = true, oym2 = true, oym1 = true, oyp1 = true;
-  if (GEN) {
-    const int a = nbr(i, -2, g.nx, g.per[0]), b = nbr(i, -1, g.nx, g.per[0]), c = nbr(i, 1, g.nx, g.per[0]);
-    oxm2 = a >= 0; oxm1 = b >= 0; oxp1 = c >= 0;
-    xm2 = q + (long long)(a - i) * sx; xm1 = q + (long long)(b - i) * sx; xp1 = q + (long long)(c - i) * sx;
-    const int d = nbr(j, -2, g.ny, g.per[1]), e = nbr(j, -1, g.ny, g.per[1]), h = nbr(j, 1, g.ny, g.per[1]);
-    oym2 = d >= 0; oym1 = e >= 0; oyp1 = h >= 0;
-    ym2 = q + (long long)(d - j) * sy; ym1 = q + (long long)(e - j) * sy; yp1 = q + (long long)(h - j) * sy;
-  } else {
-    xm2 = q - 2 * sx; xm1 = q - sx; xp1 = q + sx;
-    ym2 = q - 2 * sy; ym1 = q - sy; yp1 = q + sy;
-  }
+__global__ __launch_bounds__(BZ* MBY) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
+  __shared__ float sEx[2][MBY + 3][MLZ];  // [row j-j0+2][col k-k0+2]
+  __shared__ float sEy[2][MBY][MLZ];      // [row j-j0][col k-k0+2]
+  __shared__ float sEz[2][MBY + 3][BZ];   // [row j-j0+2][col k-k0]
+  MarchTile t;
+  if (!march_locate(ms, t)) return;
+  const Region& R = ms.r[t.r];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int j = t.j0 + ty, k = t.k0 + tx;
+  const bool inreg = j < R.j1 && k < R.k1;
+  const long long own = plane_off(g, j, k);
+  // halo positions: rows j0-2, j0-1, j0+MBY (threads ty<3); columns k0-2, k0-1, k0+64 (tx<3)
+  const int yrow = ty < 2 ? ty : MBY + 2;
+  const long long yh = ty < 3 ? plane_off(g, t.j0 + (ty < 2 ? ty - 2 : MBY), k) : -1;
+  const int zcol = tx < 2 ? tx : BZ + 2;
+  const long long zh = tx < 3 ? plane_off(g, j, t.k0 + (tx < 2 ? tx - 2 : BZ)) : -1;
   const auto* Ex = f.E[0];
   const auto* Ey = f.E[1];
   const auto* Ez = f.E[2];
-  const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
-  const float ez0 = ld(Ez, q, iez), ey0 = ld(Ey, q, iey), ex0 = ld(Ex, q, iex);
-  // derivatives of E (backward stencil: samples at -2,-1,0,+1)
-  float dyEz = g.rd[1] * d24(ldn(Ez, ym2, oym2, iez), ldn(Ez, ym1, oym1, iez), ez0, ldn(Ez, yp1, oyp1, iez));
-  float dxEz = g.rd[0] * d24(ldn(Ez, xm2, oxm2, iez), ldn(Ez, xm1, oxm1, iez), ez0, ldn(Ez, xp1, oxp1, iez));
-  float dzEy = g.rd[2] * d24(ldn(Ey, rowq + km2, km2 >= 0, iey), ldn(Ey, rowq + km1, km1 >= 0, iey), ey0,
-                             ldn(Ey, rowq + kp1, kp1 >= 0, iey));
-  float dxEy = g.rd[0] * d24(ldn(Ey, xm2, oxm2, iey), ldn(Ey, xm1, oxm1, iey), ey0, ldn(Ey, xp1, oxp1, iey));
-  float dzEx = g.rd[2] * d24(ldn(Ex, rowq + km2, km2 >= 0, iex), ldn(Ex, rowq + km1, km1 >= 0, iex), ex0,
-                             ldn(Ex, rowq + kp1, kp1 >= 0, iex));
-  float dyEx = g.rd[1] * d24(ldn(Ex, ym2, oym2, iex), ldn(Ex, ym1, oym1, iex), ex0, ldn(Ex, yp1, oyp1, iex));
-
-  if (pm.on) {
-    // z axis (all regions): slot 0 = comp x (Hx: D_z Ey), slot 1 = comp y (Hy: D_z Ex)
-    const int sz = slab_index(k, g.nz, pm.plo[2], pm.phi[2]);
-    if (sz >= 0) {
-      const float b = pm.bH[2][k], c = pm.cH[2][k];
-      const long long pq = psi_index(g, pm, 2, sz, i, j, k);
-      cpml_apply(pm.psiH[2][0], pq, b, c, dzEy);
-      cpml_apply(pm.psiH[2][1], pq, b, c, dzEx);
-    }
-    if (GEN) {
-      const int sxi = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
-      if (sxi >= 0) {  // x axis: slot 0 = comp y (Hy: D_x Ez), slot 1 = comp z (Hz: D_x Ey)
-        const float b = pm.bH[0][i], c = pm.cH[0][i];
-        const long long pq = psi_index(g, pm, 0, sxi, i, j, k);
-        cpml_apply(pm.psiH[0][0], pq, b, c, dxEz);
-        cpml_apply(pm.psiH[0][1], pq, b, c, dxEy);
-      }
-      const int syi = slab_index(j, g.ny, pm.plo[1], pm.phi[1]);
-      if (syi >= 0) {  // y axis: slot 0 = comp z (Hz: D_y Ex), slot 1 = comp x (Hx: D_y Ez)
-        const float b = pm.bH[1][j], c = pm.cH[1][j];
-        const long long pq = psi_index(g, pm, 1, syi, i, j, k);
-        cpml_apply(pm.psiH[1][0], pq, b, c, dyEx);
-        cpml_apply(pm.psiH[1][1], pq, b, c, dyEz);
-      }
-    }
-  }
   auto* Hx = f.H[0];
   auto* Hy = f.H[1];
   auto* Hz = f.H[2];
-  const float dt = g.dt;
-  st(Hx, q, ld(Hx, q, sc.ih[0]) - dt * (dyEz - dzEy), sc.sh[0]);
-  st(Hy, q, ld(Hy, q, sc.ih[1]) - dt * (dzEx - dxEz), sc.sh[1]);
-  st(Hz, q, ld(Hz, q, sc.ih[2]) - dt * (dxEy - dyEx), sc.sh[2]);
+  const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
+  // x queue: Ey, Ez at planes i-2 .. i+1
+  float qy[4], qz[4];
+  for (int s = 0; s < 4; s++) {
+    const long long pb = plane_base<GEN>(g, t.i0 - 2 + s);
+    qy[s] = ldp(Ey, pb, own, iey);
+    qz[s] = ldp(Ez, pb, own, iez);
+  }
+  const int zs = pm.on ? slab_index(k, g.nz, pm.plo[2], pm.phi[2]) : -1;
+  const int ysl = (GEN && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
+  const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
+  // plane-i operands, loaded one plane ahead: own Ex, the two halo values
+  // and the H read-modify-write operands
+  float n_ex, n_y0 = 0.f, n_y1 = 0.f, n_z0 = 0.f, n_z1 = 0.f, n_hx = 0.f, n_hy = 0.f, n_hz = 0.f;
+  {
+    const long long pb = plane_base<GEN>(g, t.i0);
+    n_ex = ldp(Ex, pb, own, iex);
+    if (ty < 3) { n_y0 = ldp(Ex, pb, yh, iex); n_y1 = ldp(Ez, pb, yh, iez); }
+    if (tx < 3) { n_z0 = ldp(Ex, pb, zh, iex); n_z1 = ldp(Ey, pb, zh, iey); }
+    if (inreg) { n_hx = ld(Hx, pb + own, sc.ih[0]); n_hy = ld(Hy, pb + own, sc.ih[1]); n_hz = ld(Hz, pb + own, sc.ih[2]); }
+  }
+  for (int i = t.i0; i < t.i1; i++) {
+    const int b = i & 1;
+    const long long pb = plane_base<GEN>(g, i);
+    const float exo = n_ex, hx0 = n_hx, hy0 = n_hy, hz0 = n_hz;
+    sEx[b][ty + 2][tx + 2] = exo;
+    sEy[b][ty][tx + 2] = qy[2];
+    sEz[b][ty + 2][tx] = qz[2];
+    if (ty < 3) {
+      sEx[b][yrow][tx + 2] = n_y0;
+      sEz[b][yrow][tx] = n_y1;
+    }
+    if (tx < 3) {
+      sEx[b][ty + 2][zcol] = n_z0;
+      sEy[b][ty][zcol] = n_z1;
+    }
+    // prefetch plane i+1 operands and the queue's plane i+2
+    float nqy = 0.f, nqz = 0.f;
+    if (i + 1 < t.i1) {
+      const long long p1 = plane_base<GEN>(g, i + 1);
+      const long long p2 = plane_base<GEN>(g, i + 2);
+      n_ex = ldp(Ex, p1, own, iex);
+      if (ty < 3) { n_y0 = ldp(Ex, p1, yh, iex); n_y1 = ldp(Ez, p1, yh, iez); }
+      if (tx < 3) { n_z0 = ldp(Ex, p1, zh, iex); n_z1 = ldp(Ey, p1, zh, iey); }
+      if (inreg) { n_hx = ld(Hx, p1 + own, sc.ih[0]); n_hy = ld(Hy, p1 + own, sc.ih[1]); n_hz = ld(Hz, p1 + own, sc.ih[2]); }
+      nqy = ldp(Ey, p2, own, iey);
+      nqz = ldp(Ez, p2, own, iez);
+    }
+    lds_barrier();
+    if (inreg) {
+      float dyEz = rd1 * d24(sEz[b][ty][tx], sEz[b][ty + 1][tx], qz[2], sEz[b][ty + 3][tx]);
+      float dyEx = rd1 * d24(sEx[b][ty][tx + 2], sEx[b][ty + 1][tx + 2], exo, sEx[b][ty + 3][tx + 2]);
+      float dzEy = rd2 * d24(sEy[b][ty][tx], sEy[b][ty][tx + 1], qy[2], sEy[b][ty][tx + 3]);
+      float dzEx = rd2 * d24(sEx[b][ty + 2][tx], sEx[b][ty + 2][tx + 1], exo, sEx[b][ty + 2][tx + 3]);
+      float dxEz = rd0 * d24(qz[0], qz[1], qz[2], qz[3]);
+      float dxEy = rd0 * d24(qy[0], qy[1], qy[2], qy[3]);
+      if (zs >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
+        const float bb = pm.bH[2][k], cc = pm.cH[2][k];
+        const long long pq = psi_index(g, pm, 2, zs, i, j, k);
+        cpml_apply(pm.psiH[2][0], pq, bb, cc, dzEy);
+        cpml_apply(pm.psiH[2][1], pq, bb, cc, dzEx);
+      }
+      if (GEN && pm.on) {
+        const int xsl = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
+        if (xsl >= 0) {  // x axis: slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
+          const float bb = pm.bH[0][i], cc = pm.cH[0][i];
+          const long long pq = psi_index(g, pm, 0, xsl, i, j, k);
+          cpml_apply(pm.psiH[0][0], pq, bb, cc, dxEz);
+          cpml_apply(pm.psiH[0][1], pq, bb, cc, dxEy);
+        }
+        if (ysl >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
+          const float bb = pm.bH[1][j], cc = pm.cH[1][j];
+          const long long pq = psi_index(g, pm, 1, ysl, i, j, k);
+          cpml_apply(pm.psiH[1][0], pq, bb, cc, dyEx);
+          cpml_apply(pm.psiH[1][1], pq, bb, cc, dyEz);
+        }
+      }
+      const long long q = pb + own;
+      st(Hx, q, hx0 - dt * (dyEz - dzEy), sc.sh[0]);
+      st(Hy, q, hy0 - dt * (dzEx - dxEz), sc.sh[1]);
+      st(Hz, q, hz0 - dt * (dxEy - dyEx), sc.sh[2]);
+    }
+    // advance the queue to planes i-1 .. i+2
+    qy[0] = qy[1]; qy[1] = qy[2]; qy[2] = qy[3]; qy[3] = nqy;
+    qz[0] = qz[1]; qz[1] = qz[2]; qz[2] = qz[3]; qz[3] = nqz;
+  }
 }
 
 // ============================================================ ADE cells
@@ -450,98 +524,141 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
 
 // ================================================================ E sweep
 template <int STORE, bool GEN>
-__global__ __launch_bounds__(256) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
-                                                  Ade ade, RegionSet rs) {
+__global__ __launch_bounds__(BZ* MBY) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
+                                                      Ade ade, MarchSet ms) {
   __shared__ ClassEntry tbl[kMaxClasses];
-  const int tid = threadIdx.x + threadIdx.y * BZ;
-  for (int t = tid; t < mt.nclass; t += BZ * BY) tbl[t] = mt.table[t];
-  __syncthreads();
-  int i, j, k;
-  if (!locate(rs, i, j, k)) return;
-  const long long sx = g.sx, sy = g.sy;
-  const long long q = i * sx + j * sy + k;
-  // z neighbours (offsets -1, +1, +2)
-  const int km1 = nbr(k, -1, g.nz, g.per[2]), kp1 = nbr(k, 1, g.nz, g.per[2]), kp2 = nbr(k, 2, g.nz, g.per[2]);
-  const long long rowq = q - k;
-  long long xm1, xp1, xp2, ym1, yp1, yp2;
-  bool oxm1 = true, oxp1 = true, oxp2 = true, oym1 = true, oyp1 = true, oyp2 = true;
-  if (GEN) {
-    const int a = nbr(i, -1, g.nx, g.per[0]), b = nbr(i, 1, g.nx, g.per[0]), c = nbr(i, 2, g.nx, g.per[0]);
-    oxm1 = a >= 0; oxp1 = b >= 0; oxp2 = c >= 0;
-    xm1 = q + (long long)(a - i) * sx; xp1 = q + (long long)(b - i) * sx; xp2 = q + (long long)(c - i) * sx;
-    const int d = nbr(j, -1, g.ny, g.per[1]), e = nbr(j, 1, g.ny, g.per[1]), h = nbr(j, 2, g.ny, g.per[1]);
-    oym1 = d >= 0; oyp1 = e >= 0; oyp2 = h >= 0;
-    ym1 = q + (long long)(d - j) * sy; yp1 = q + (long long)(e - j) * sy; yp2 = q + (long long)(h - j) * sy;
-  } else {
-    xm1 = q - sx; xp1 = q + sx; xp2 = q + 2 * sx;
-    ym1 = q - sy; yp1 = q + sy; yp2 = q + 2 * sy;
-  }
+  __shared__ float sHx[2][MBY + 3][MLZ];  // [row j-j0+1][col k-k0+1]
+  __shared__ float sHy[2][MBY][MLZ];      // [row j-j0][col k-k0+1]
+  __shared__ float sHz[2][MBY + 3][BZ];   // [row j-j0+1][col k-k0]
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int q = tx + ty * BZ; q < mt.nclass; q += BZ * MBY) tbl[q] = mt.table[q];
+  MarchTile t;
+  if (!march_locate(ms, t)) return;  // block-uniform
+  const Region& R = ms.r[t.r];
+  const int j = t.j0 + ty, k = t.k0 + tx;
+  const bool inreg = j < R.j1 && k < R.k1;
+  const long long own = plane_off(g, j, k);
+  // halo rows j0-1, j0+MBY, j0+MBY+1 (ty<3); columns k0-1, k0+64, k0+65 (tx<3)
+  const int yrow = ty == 0 ? 0 : MBY + ty;
+  const long long yh = ty < 3 ? plane_off(g, t.j0 + (ty == 0 ? -1 : MBY + ty - 1), k) : -1;
+  const int zcol = tx == 0 ? 0 : BZ + tx;
+  const long long zh = tx < 3 ? plane_off(g, j, t.k0 + (tx == 0 ? -1 : BZ + tx - 1)) : -1;
   const auto* Hx = f.H[0];
   const auto* Hy = f.H[1];
   const auto* Hz = f.H[2];
-  const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
-  const float hz0 = ld(Hz, q, ihz), hy0 = ld(Hy, q, ihy), hx0 = ld(Hx, q, ihx);
-  // derivatives of H (forward stencil: samples at -1,0,+1,+2)
-  float dyHz = g.rd[1] * d24(ldn(Hz, ym1, oym1, ihz), hz0, ldn(Hz, yp1, oyp1, ihz), ldn(Hz, yp2, oyp2, ihz));
-  float dxHz = g.rd[0] * d24(ldn(Hz, xm1, oxm1, ihz), hz0, ldn(Hz, xp1, oxp1, ihz), ldn(Hz, xp2, oxp2, ihz));
-  float dzHy = g.rd[2] * d24(ldn(Hy, rowq + km1, km1 >= 0, ihy), hy0, ldn(Hy, rowq + kp1, kp1 >= 0, ihy),
-                             ldn(Hy, rowq + kp2, kp2 >= 0, ihy));
-  float dxHy = g.rd[0] * d24(ldn(Hy, xm1, oxm1, ihy), hy0, ldn(Hy, xp1, oxp1, ihy), ldn(Hy, xp2, oxp2, ihy));
-  float dzHx = g.rd[2] * d24(ldn(Hx, rowq + km1, km1 >= 0, ihx), hx0, ldn(Hx, rowq + kp1, kp1 >= 0, ihx),
-                             ldn(Hx, rowq + kp2, kp2 >= 0, ihx));
-  float dyHx = g.rd[1] * d24(ldn(Hx, ym1, oym1, ihx), hx0, ldn(Hx, yp1, oyp1, ihx), ldn(Hx, yp2, oyp2, ihx));
-
-  if (pm.on) {
-    const int sz = slab_index(k, g.nz, pm.plo[2], pm.phi[2]);
-    if (sz >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
-      const float b = pm.bE[2][k], c = pm.cE[2][k];
-      const long long pq = psi_index(g, pm, 2, sz, i, j, k);
-      cpml_apply(pm.psiE[2][0], pq, b, c, dzHy);
-      cpml_apply(pm.psiE[2][1], pq, b, c, dzHx);
-    }
-    if (GEN) {
-      const int sxi = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
-      if (sxi >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
-        const float b = pm.bE[0][i], c = pm.cE[0][i];
-        const long long pq = psi_index(g, pm, 0, sxi, i, j, k);
-        cpml_apply(pm.psiE[0][0], pq, b, c, dxHz);
-        cpml_apply(pm.psiE[0][1], pq, b, c, dxHy);
-      }
-      const int syi = slab_index(j, g.ny, pm.plo[1], pm.phi[1]);
-      if (syi >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
-        const float b = pm.bE[1][j], c = pm.cE[1][j];
-        const long long pq = psi_index(g, pm, 1, syi, i, j, k);
-        cpml_apply(pm.psiE[1][0], pq, b, c, dyHx);
-        cpml_apply(pm.psiE[1][1], pq, b, c, dyHz);
-      }
-    }
-  }
-  const float cx = dyHz - dzHy, cy = dzHx - dxHz, cz = dxHy - dyHx;
-  const ClassEntry& ce = tbl[mt.cls ? mt.cls[q] : 0];
-  float ie[3] = {ce.inv_eps[0], ce.inv_eps[1], ce.inv_eps[2]};
-  bool over = false;
-  if (mt.d_on) {
-    const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
-    if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) {
-      const float v = mt.d_inv_eps[((long long)di * mt.d_nj + dj) * mt.d_nk + dk];
-      ie[0] = ie[1] = ie[2] = v;
-      over = true;
-    }
-  }
   auto* Ex = f.E[0];
   auto* Ey = f.E[1];
   auto* Ez = f.E[2];
-  if (ce.special && !over) {  // finished by k_ade
-    const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
-    ade.curl[0][b] = cx;
-    ade.curl[1][b] = cy;
-    ade.curl[2][b] = cz;
-    return;
+  const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
+  // x queue: Hy, Hz at planes i-1 .. i+2
+  float qy[4], qz[4];
+  for (int s = 0; s < 4; s++) {
+    const long long pb = plane_base<GEN>(g, t.i0 - 1 + s);
+    qy[s] = ldp(Hy, pb, own, ihy);
+    qz[s] = ldp(Hz, pb, own, ihz);
   }
-  const float dt = g.dt;
-  st(Ex, q, ld(Ex, q, sc.ie[0]) + dt * ie[0] * cx, sc.se[0]);
-  st(Ey, q, ld(Ey, q, sc.ie[1]) + dt * ie[1] * cy, sc.se[1]);
-  st(Ez, q, ld(Ez, q, sc.ie[2]) + dt * ie[2] * cz, sc.se[2]);
+  const int zs = pm.on ? slab_index(k, g.nz, pm.plo[2], pm.phi[2]) : -1;
+  const int ysl = (GEN && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
+  const bool design_col = mt.d_on && j >= mt.d_j0 && j < mt.d_j0 + mt.d_nj && k >= mt.d_k0 && k < mt.d_k0 + mt.d_nk;
+  const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
+  // plane-i operands, loaded one plane ahead: own Hx, halo values, the E
+  // read-modify-write operands and the material class byte
+  float n_hx, n_y0 = 0.f, n_y1 = 0.f, n_z0 = 0.f, n_z1 = 0.f, n_ex = 0.f, n_ey = 0.f, n_ez = 0.f;
+  int n_cls = 0;
+  {
+    const long long pb = plane_base<GEN>(g, t.i0);
+    n_hx = ldp(Hx, pb, own, ihx);
+    if (ty < 3) { n_y0 = ldp(Hx, pb, yh, ihx); n_y1 = ldp(Hz, pb, yh, ihz); }
+    if (tx < 3) { n_z0 = ldp(Hx, pb, zh, ihx); n_z1 = ldp(Hy, pb, zh, ihy); }
+    if (inreg) {
+      n_ex = ld(Ex, pb + own, sc.ie[0]); n_ey = ld(Ey, pb + own, sc.ie[1]); n_ez = ld(Ez, pb + own, sc.ie[2]);
+      n_cls = mt.cls ? mt.cls[pb + own] : 0;
+    }
+  }
+  for (int i = t.i0; i < t.i1; i++) {
+    const int b = i & 1;
+    const long long pb = plane_base<GEN>(g, i);
+    const float hxo = n_hx, ex0 = n_ex, ey0 = n_ey, ez0 = n_ez;
+    const int cls = n_cls;
+    sHx[b][ty + 1][tx + 1] = hxo;
+    sHy[b][ty][tx + 1] = qy[1];
+    sHz[b][ty + 1][tx] = qz[1];
+    if (ty < 3) {
+      sHx[b][yrow][tx + 1] = n_y0;
+      sHz[b][yrow][tx] = n_y1;
+    }
+    if (tx < 3) {
+      sHx[b][ty + 1][zcol] = n_z0;
+      sHy[b][ty][zcol] = n_z1;
+    }
+    // prefetch plane i+1 operands and the queue's plane i+3
+    float nqy = 0.f, nqz = 0.f;
+    if (i + 1 < t.i1) {
+      const long long p1 = plane_base<GEN>(g, i + 1);
+      const long long p3 = plane_base<GEN>(g, i + 3);
+      n_hx = ldp(Hx, p1, own, ihx);
+      if (ty < 3) { n_y0 = ldp(Hx, p1, yh, ihx); n_y1 = ldp(Hz, p1, yh, ihz); }
+      if (tx < 3) { n_z0 = ldp(Hx, p1, zh, ihx); n_z1 = ldp(Hy, p1, zh, ihy); }
+      if (inreg) {
+        n_ex = ld(Ex, p1 + own, sc.ie[0]); n_ey = ld(Ey, p1 + own, sc.ie[1]); n_ez = ld(Ez, p1 + own, sc.ie[2]);
+        n_cls = mt.cls ? mt.cls[p1 + own] : 0;
+      }
+      nqy = ldp(Hy, p3, own, ihy);
+      nqz = ldp(Hz, p3, own, ihz);
+    }
+    lds_barrier();
+    if (inreg) {
+      float dyHz = rd1 * d24(sHz[b][ty][tx], qz[1], sHz[b][ty + 2][tx], sHz[b][ty + 3][tx]);
+      float dyHx = rd1 * d24(sHx[b][ty][tx + 1], hxo, sHx[b][ty + 2][tx + 1], sHx[b][ty + 3][tx + 1]);
+      float dzHy = rd2 * d24(sHy[b][ty][tx], qy[1], sHy[b][ty][tx + 2], sHy[b][ty][tx + 3]);
+      float dzHx = rd2 * d24(sHx[b][ty + 1][tx], hxo, sHx[b][ty + 1][tx + 2], sHx[b][ty + 1][tx + 3]);
+      float dxHz = rd0 * d24(qz[0], qz[1], qz[2], qz[3]);
+      float dxHy = rd0 * d24(qy[0], qy[1], qy[2], qy[3]);
+      if (zs >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
+        const float bb = pm.bE[2][k], cc = pm.cE[2][k];
+        const long long pq = psi_index(g, pm, 2, zs, i, j, k);
+        cpml_apply(pm.psiE[2][0], pq, bb, cc, dzHy);
+        cpml_apply(pm.psiE[2][1], pq, bb, cc, dzHx);
+      }
+      if (GEN && pm.on) {
+        const int xsl = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
+        if (xsl >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
+          const float bb = pm.bE[0][i], cc = pm.cE[0][i];
+          const long long pq = psi_index(g, pm, 0, xsl, i, j, k);
+          cpml_apply(pm.psiE[0][0], pq, bb, cc, dxHz);
+          cpml_apply(pm.psiE[0][1], pq, bb, cc, dxHy);
+        }
+        if (ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
+          const float bb = pm.bE[1][j], cc = pm.cE[1][j];
+          const long long pq = psi_index(g, pm, 1, ysl, i, j, k);
+          cpml_apply(pm.psiE[1][0], pq, bb, cc, dyHx);
+          cpml_apply(pm.psiE[1][1], pq, bb, cc, dyHz);
+        }
+      }
+      const float cx = dyHz - dzHy, cy = dzHx - dxHz, cz = dxHy - dyHx;
+      const long long q = pb + own;
+      const ClassEntry& ce = tbl[cls];
+      float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
+      bool over = false;
+      if (design_col && i >= mt.d_i0 && i < mt.d_i0 + mt.d_ni) {
+        const float v = mt.d_inv_eps[((long long)(i - mt.d_i0) * mt.d_nj + (j - mt.d_j0)) * mt.d_nk + (k - mt.d_k0)];
+        ie0 = ie1 = ie2 = v;
+        over = true;
+      }
+      if (ce.special && !over) {  // finished by k_ade
+        const long long bq = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
+        ade.curl[0][bq] = cx;
+        ade.curl[1][bq] = cy;
+        ade.curl[2][bq] = cz;
+      } else {
+        st(Ex, q, ex0 + dt * ie0 * cx, sc.se[0]);
+        st(Ey, q, ey0 + dt * ie1 * cy, sc.se[1]);
+        st(Ez, q, ez0 + dt * ie2 * cz, sc.se[2]);
+      }
+    }
+    qy[0] = qy[1]; qy[1] = qy[2]; qy[2] = qy[3]; qy[3] = nqy;
+    qz[0] = qz[1]; qz[1] = qz[2]; qz[2] = qz[3]; qz[3] = nqz;
+  }
 }
 
 // ======================================================== small kernels
@@ -639,17 +756,34 @@ static void check_launch(const char* what) {
   if (e != hipSuccess) fprintf(stderr, "dtm: launch of %s failed: %s\n", what, hipGetErrorString(e));
 }
 
+static int march_seg(const Region* regs, int nreg) {
+  // Segments long enough to amortise the 3-plane queue warm-up, short enough
+  // to give every CU several blocks.
+  long long cols = 0;
+  int maxlen = 1;
+  for (int t = 0; t < nreg; t++) {
+    const Region& r = regs[t];
+    if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
+    cols += (long long)((r.k1 - r.k0 + BZ - 1) / BZ) * ((r.j1 - r.j0 + MBY - 1) / MBY);
+    maxlen = std::max(maxlen, r.i1 - r.i0);
+  }
+  int seg = 32;
+  while (seg > 8 && cols * ((maxlen + seg - 1) / seg) < 4096) seg /= 2;
+  return seg;
+}
+
 template <int STORE>
 static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
                   const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
-  const dim3 blk(BZ, BY, 1);
-  RegionSet rm = make_set(&main, main.i1 > main.i0 ? 1 : 0);
+  const dim3 blk(BZ, MBY, 1);
+  const int nm = main.i1 > main.i0 ? 1 : 0;
+  MarchSet rm = make_march(&main, nm, march_seg(&main, nm));
   if (rm.total > 0) {
     hipLaunchKernelGGL((k_update_h<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, rm);
     check_launch("k_update_h<main>");
   }
-  RegionSet rg = make_set(regs, nreg);
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg));
   if (rg.total > 0) {
     hipLaunchKernelGGL((k_update_h<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
     check_launch("k_update_h<edge>");
@@ -660,13 +794,14 @@ template <int STORE>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                   const Ade& ade, const Region* regs, int nreg, const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
-  const dim3 blk(BZ, BY, 1);
-  RegionSet rm = make_set(&main, main.i1 > main.i0 ? 1 : 0);
+  const dim3 blk(BZ, MBY, 1);
+  const int nm = main.i1 > main.i0 ? 1 : 0;
+  MarchSet rm = make_march(&main, nm, march_seg(&main, nm));
   if (rm.total > 0) {
     hipLaunchKernelGGL((k_update_e<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rm);
     check_launch("k_update_e<main>");
   }
-  RegionSet rg = make_set(regs, nreg);
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg));
   if (rg.total > 0) {
     hipLaunchKernelGGL((k_update_e<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
     check_launch("k_update_e<edge>");

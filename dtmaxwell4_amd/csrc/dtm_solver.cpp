@@ -310,13 +310,16 @@ static int setup_tfsf(dtm_solver* s) {
       t.chi[a] = 0;
       continue;
     }
-    // a neighbour at half-offset within +-3 can cross a face only near it
-    t.clo[a] = (t.hlo[a] - 4) / 2 - 1;
-    t.chi[a] = (t.hhi[a] + 4) / 2 + 1;
-    t.fl0[a] = (t.hlo[a] - 4) / 2 - 1;
-    t.fl1[a] = (t.hlo[a] + 3) / 2 + 1;
-    t.fh0[a] = (t.hhi[a] - 4) / 2 - 1;
-    t.fh1[a] = (t.hhi[a] + 4) / 2 + 1;
+    // Cells whose stencil (half-offsets +-1, +-3 along a) can cross a face:
+    // component half-coordinate h in [hlo-3, hlo+2] or [hhi-2, hhi+3], and
+    // on the other axes h inside [hlo, hhi] (h = 2i or 2i+1).
+    auto fdiv = [](int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); };
+    t.clo[a] = fdiv(t.hlo[a]);
+    t.chi[a] = fdiv(t.hhi[a]);
+    t.fl0[a] = fdiv(t.hlo[a] - 3);
+    t.fl1[a] = fdiv(t.hlo[a] + 2);
+    t.fh0[a] = fdiv(t.hhi[a] - 2);
+    t.fh1[a] = fdiv(t.hhi[a] + 3);
   }
   t.w = c.tfsf.w;
   t.kEnv = c.tfsf.kEnv;
