@@ -78,7 +78,13 @@ def sphere_mats(shape, d, centre, radius, mat_id, base=None):
 def seeded_points(n, shape, margin, avoid, avoid_r, seed=1234):
     rng = np.random.RandomState(seed)  # MT19937
     pts = []
+    if any(s - 2 * margin <= 0 for s in shape):
+        raise ValueError("no room for sensors: grid too small for the margin")
+    tries = 0
     while len(pts) < n:
+        tries += 1
+        if tries > 100000:
+            raise ValueError("could not place sensors away from the source")
         p = [int(rng.randint(margin, s - margin)) for s in shape]
         if sum((p[a] - avoid[a]) ** 2 for a in range(3)) >= avoid_r ** 2:
             pts.append(tuple(p))

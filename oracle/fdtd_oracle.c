@@ -975,6 +975,12 @@ int orc_step(orc_sim* s, int nsteps) {
 
 long long orc_time_step(const orc_sim* s) { return s->step; }
 
+/* Current value of a reference sensor (kind 1) or raw probe (kind 0). */
+double orc_probe_now(const orc_sim* s, int kind, int comp, int i, int j, int k) {
+  if (kind == 0) return comp < 3 ? s->E[comp][IDX(s, i, j, k)] : s->H[comp - 3][IDX(s, i, j, k)];
+  return sensor_value(s, comp, i, j, k);
+}
+
 int orc_field_read(const orc_sim* s, int comp, double* out) {
   const real* F = comp < 3 ? s->E[comp] : s->H[comp - 3];
   for (size_t q = 0; q < s->n; q++) out[q] = (double)F[q];

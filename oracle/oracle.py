@@ -44,6 +44,7 @@ def lib(fp32=False):
         "orc_probe_read": (C.c_int, [P, P, C.c_longlong]),
         "orc_dft_read": (C.c_int, [P, P]),
         "orc_nthreads": (C.c_int, []),
+        "orc_probe_now": (C.c_double, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "orc_real_size": (C.c_int, []),
         "orc_incident": (C.c_double, [C.POINTER(Tfsf), C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double]),
@@ -135,6 +136,9 @@ class Oracle:
 
     def fields(self):
         return [self.field(c) for c in range(6)]
+
+    def probe_now(self, kind, comp, i, j, k):
+        return self.L.orc_probe_now(self.h, kind, comp, i, j, k)
 
     def probes(self):
         rows = self.time_step

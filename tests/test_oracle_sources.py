@@ -1,0 +1,204 @@
+"""Pins the oracle's sources and outputs:
+
+* the dipole incident field (signal.hpp:51-97) satisfies Maxwell's equations
+  in vacuum (curl E = -dB/dt, curl B = dE/dt / c^2) and the far-field
+  identities E.n -> 0, |E| -> c|B|; the plane wave (signal.hpp:14-50) has
+  E perpendicular to k, H = k x E / Imp;
+* TF/SF corrections (genAlg.py:207-234) cancel the incident wave outside a
+  plane-wave box and, for the reference's dipole (scattered field inside),
+  leave the box interior empty while the exterior carries the analytic
+  dipole field;
+* reference sensors (sensor.h:64-71) return the node value of a linear
+  field;
+* the running DFT uses sample times t = (n+1)dt for E and (n+1/2)dt for H
+  and the dt weight;
+* the Purcell expression (purcell.gp:2) reproduces the closed-form Purcell
+  factor of a dipole over a perfect mirror (image dipole) under the e^{+iwt}
+  transform convention of the DFT.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from dtmaxwell4_amd import abi, configs
+from oracle import oracle as orc
+from oracle.oracle import Oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def dipole_src(waveform):
+    t = abi.Tfsf()
+    t.kind, t.waveform, t.sf_inside = abi.TFSF_DIPOLE, waveform, 1
+    t.box[:] = [-0.1, 0.1, -0.1, 0.1, -0.1, 0.1]
+    t.w, t.kEnv, t.cSrc, t.amp = 2 * np.pi / 0.8, 0.1, 0.7, 1.0
+    t.t0, t.tw = 1.5, 0.4
+    return t
+
+
+@pytest.mark.parametrize("waveform", [abi.WAVE_RAISED_COS, abi.WAVE_GAUSS_SINE])
+def test_dipole_field_satisfies_maxwell(waveform):
+    src = dipole_src(waveform)
+    c = src.cSrc
+    f = lambda comp, r, t: orc.incident(src, comp, r[0], r[1], r[2], t)
+    h = 1e-4
+    for r0, t0 in [((0.31, -0.22, 0.47), 2.1), ((-0.8, 0.5, 0.2), 3.0)]:
+        r0 = np.array(r0)
+
+        def grad(comp, t):
+            g = np.zeros(3)
+            for a in range(3):
+                e = np.zeros(3)
+                e[a] = h
+                g[a] = (f(comp, r0 + e, t) - f(comp, r0 - e, t)) / (2 * h)
+            return g
+
+        dE = [grad(q, t0) for q in range(3)]
+        dB = [grad(3 + q, t0) for q in range(3)]
+        curlE = np.array([dE[2][1] - dE[1][2], dE[0][2] - dE[2][0], dE[1][0] - dE[0][1]])
+        curlB = np.array([dB[2][1] - dB[1][2], dB[0][2] - dB[2][0], dB[1][0] - dB[0][1]])
+        dBdt = np.array([(f(3 + q, r0, t0 + h) - f(3 + q, r0, t0 - h)) / (2 * h) for q in range(3)])
+        dEdt = np.array([(f(q, r0, t0 + h) - f(q, r0, t0 - h)) / (2 * h) for q in range(3)])
+        s1 = np.linalg.norm(curlE) + 1e-30
+        s2 = np.linalg.norm(curlB) + 1e-30
+        assert np.linalg.norm(curlE + dBdt) < 1e-5 * s1
+        assert np.linalg.norm(curlB - dEdt / c ** 2) < 1e-5 * s2
+
+
+def test_dipole_far_field_identities():
+    src = dipole_src(abi.WAVE_GAUSS_SINE)
+    n = np.array([0.6, -0.48, 0.64])
+    n /= np.linalg.norm(n)
+    R = 400.0
+    t = R / src.cSrc + src.t0
+    E = np.array([orc.incident(src, q, *(R * n), t) for q in range(3)])
+    B = np.array([orc.incident(src, 3 + q, *(R * n), t) for q in range(3)])
+    assert abs(E @ n) < 1e-2 * np.linalg.norm(E)
+    assert np.linalg.norm(E) == pytest.approx(src.cSrc * np.linalg.norm(B), rel=1e-2)
+
+
+def test_plane_wave_transversality():
+    t = abi.Tfsf()
+    t.kind, t.waveform = abi.TFSF_PLANE, abi.WAVE_GAUSS_SINE
+    t.w, t.cSrc, t.Imp, t.amp, t.t0, t.tw = 5.0, 0.8, 0.8, 1.0, 1.0, 0.5
+    t.theta, t.phi, t.psi = 0.7, 1.1, 0.4
+    k = np.array([np.sin(0.7) * np.cos(1.1), np.sin(0.7) * np.sin(1.1), np.cos(0.7)])
+    for r, tt in [((0.1, 0.2, 0.3), 1.0), ((0.5, -0.2, 0.1), 1.3)]:
+        E = np.array([orc.incident(t, q, *r, tt) for q in range(3)])
+        H = np.array([orc.incident(t, 3 + q, *r, tt) for q in range(3)])
+        assert abs(E @ k) < 1e-12 * (np.linalg.norm(E) + 1e-30)
+        np.testing.assert_allclose(H, np.cross(k, E) / t.Imp, atol=1e-12)
+
+
+def test_plane_wave_tfsf_no_leakage_cpu():
+    w = configs.plane_wave_sphere(nx=56, ny=36, nz=36, npml=8, inset=4, radius=0, eps=1.0, steps=160,
+                                  periods_w=1.0, periods_t0=2.0)
+    o = configs.apply(w, Oracle(w.cfg))
+    o.step(160)
+    ez = o.field(abi.EZ)
+    b = 12
+    inside = np.abs(ez[b + 3:-b - 3, b + 3:-b - 3, b + 3:-b - 3]).max()
+    outside = max(np.abs(ez[:b - 3]).max(), np.abs(ez[:, :b - 3]).max(), np.abs(ez[:, :, :b - 3]).max())
+    assert inside > 0.2
+    assert outside < 3e-3 * inside
+
+
+def test_reference_dipole_tfsf_is_scattered_inside_total_outside():
+    """The reference's convention: inSF() is the box interior and the
+    correction is +inc when the updated point is outside and its neighbour
+    inside.  Without a scatterer the interior stays empty and the exterior
+    carries the incident dipole field."""
+    n, d = 36, 0.05
+    cfg = abi.default_config(n, n, n, d, d, d, configs.dt_for(d, d, d))
+    cfg.npml[:] = [8] * 6
+    t = cfg.tfsf
+    t.kind, t.waveform, t.sf_inside = abi.TFSF_DIPOLE, abi.WAVE_GAUSS_SINE, 1
+    c0 = n * d / 2
+    hb = 4.0 * d
+    t.box[:] = [c0 - hb, c0 + hb, c0 - hb, c0 + hb, c0 - hb, c0 + hb]
+    t.w, t.cSrc, t.Imp, t.amp = 2 * np.pi / 0.6, 1.0, 1.0, 1e-3
+    t.tw, t.t0 = 0.25, 0.9
+    t.tStop = 1e30
+    o = Oracle(cfg)
+    steps = 110
+    o.step(steps)
+    ez = o.field(abi.EZ)
+    ci = n // 2
+    inner = np.abs(ez[ci - 2:ci + 2, ci - 2:ci + 2, ci - 2:ci + 2]).max()
+    # exterior point on the x axis, 7 cells from the centre
+    i = ci + 7
+    x, y, z = (i + 0.5) * d, (ci + 0.5) * d, ci * d
+    T = steps * cfg.dt
+    ana = orc.incident(t, abi.EZ, x, y, z, T)
+    num = ez[i, ci, ci]
+    assert abs(ana) > 1e-3
+    assert inner < 2e-2 * abs(ana)
+    assert num == pytest.approx(ana, rel=0.08)
+
+
+def test_sensor_interpolation_is_exact_for_linear_fields():
+    n, d = (10, 9, 8), 0.1
+    cfg = abi.default_config(*n, d, d, d, configs.dt_for(d, d, d))
+    o = Oracle(cfg)
+    coef = {c: np.array([0.3 + c, -1.1 * c + 0.2, 0.7, 2.0 - 0.5 * c]) for c in range(6)}
+    for c in range(6):
+        x, y, z = configs.component_positions(n, c, d, d, d)
+        a = coef[c]
+        o.set_field(c, (a[0] + a[1] * x + a[2] * y + a[3] * z) * np.ones(n))
+    for comp in range(6):
+        for (xi, yi, zi) in [(3, 4, 2), (5, 2, 5), (1, 1, 1)]:
+            v = o.probe_now(1, comp, xi, yi, zi)
+            a = coef[comp]
+            X, Y, Z = xi * d, yi * d, zi * d  # the reference's Yee node (2x,2y,2z)
+            assert v == pytest.approx(a[0] + a[1] * X + a[2] * Y + a[3] * Z, abs=1e-12)
+    # z == 0 uses the reference's mirror (sensor.h:64): zM = z+1 for E with z offsets
+    v = o.probe_now(1, abi.EX, 3, 4, 0)
+    a = coef[0]
+    # samples at z half-coords 1 and 3: (z=0.05 and z=0.15) average = 0.1
+    assert v == pytest.approx(a[0] + a[1] * 0.3 + a[2] * 0.4 + a[3] * 0.1, abs=1e-12)
+
+
+def test_dft_uses_sample_times_and_dt_weight():
+    w = configs.vacuum_dipole(n=32, npml=4, steps=120)
+    om = np.array([2 * np.pi, 5.0])
+    o = configs.apply(w, Oracle(w.cfg))
+    o.add_probe(abi.HY, 18, 16, 16)
+    o.set_dft(om)
+    o.step(120)
+    ser = o.probes()
+    dft = o.dft()
+    dt = w.cfg.dt
+    n = np.arange(120)
+    for p in range(ser.shape[1]):
+        comp = abi.EZ if p < 8 else abi.HY
+        t = (n + 1.0) * dt if comp < 3 else (n + 0.5) * dt
+        ref = (ser[:, p][:, None] * np.exp(1j * t[:, None] * om[None, :]) * dt).sum(0)
+        np.testing.assert_allclose(dft[p], ref, rtol=1e-12, atol=1e-18)
+
+
+def test_purcell_expression_matches_image_dipole_closed_form():
+    """P = 1 + 3[sin x / x^3 - cos x / x^2], x = 2kh, for a dipole normal to a
+    perfect mirror at height h; spectra in the e^{+iwt} convention
+    (F(w) = sum f(t) e^{+iwt} dt, the convention of the device DFT)."""
+    c = 1.0
+    for w, h in [(3.0, 0.2), (10.0, 0.05), (7.0, 0.31)]:
+        k = w / c
+        R = 2 * h
+        d = 0.7 * np.exp(0.4j)                           # e^{-iwt} amplitude
+        Es = 2 * d * (1 / R ** 3 - 1j * k / R ** 2) * np.exp(1j * k * R)  # image field, on axis
+        # e^{+iwt} convention: conjugate amplitudes
+        Ep, dp = np.conj(Es), np.conj(d)
+        P = orc.purcell(w, abs(Ep), np.angle(Ep), abs(dp), np.angle(dp))
+        x = k * R
+        assert P == pytest.approx(1 + 3 * (np.sin(x) / x ** 3 - np.cos(x) / x ** 2), rel=1e-12)
+
+
+def test_purcell_fixture_rows_regression():
+    """The reference's own spectra (fE.dat, fdip.dat) through the expression:
+    regression against the committed rows."""
+    data = json.load(open(os.path.join(HERE, "golden", "purcell_rows.json")))
+    assert len(data["rows"]) == 42
+    for r in data["rows"]:
+        assert orc.purcell(r["omega"], r["AE"], r["phE"], r["Ad"], r["phd"]) == pytest.approx(r["P"], rel=1e-12)
