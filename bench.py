@@ -25,23 +25,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-GROUPS = ("H interior", "H edge slabs", "E interior", "E edge slabs", "sources+probes")
-# algorithmic bytes per cell of one half-step sweep: read 3E+3H, write 3 comps
+GROUPS_TWOPASS = ("H interior", "H edge slabs", "E interior", "E edge slabs", "sources+probes")
+GROUPS_FUSED = ("step interior", "step edge slabs", "-", "-", "TF/SF+ADE+sources+probes")
+# Algorithmic bytes per cell.  Two-pass half-step sweep: read 3E+3H, write 3
+# components (+1 class byte for E).  Fused step (H and E in one pass): read
+# 3E+3H, write 3E+3H, read the class byte; cells inside the z CPML slabs also
+# read and write four FP32 psi values (two H, two E).
 WORDS_PER_HALF = 9
+WORDS_FUSED = 12
+PSI_BYTES_Z = 8 * 4
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_run(storage, steps, warmup, nx, ny, nz):
+def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
     import numpy as np
 
     from dtmaxwell4_amd import abi, configs
     from dtmaxwell4_amd.solver import Solver
     w = configs.plane_wave_sphere(nx=nx, ny=ny, nz=nz, storage=storage)
+    w.cfg.sweep = sweep
     s = configs.apply(w, Solver(w.cfg))
-    log(f"[bench] {w.name} {nx}x{ny}x{nz} storage={'fp16' if storage else 'fp32'} warmup={warmup} steps={steps}")
+    log(f"[bench] {w.name} {nx}x{ny}x{nz} storage={'fp16' if storage else 'fp32'} "
+        f"sweep={'fused' if sweep else 'twopass'} warmup={warmup} steps={steps}")
     s.step(warmup)
     s.sync()
     s.set_kernel_timing(True)
@@ -58,18 +66,26 @@ def gpu_run(storage, steps, warmup, nx, ny, nz):
     if not np.isfinite(ez).all():
         raise RuntimeError("non-finite field after timed run")
     esz = 2 if storage else 4
+    fused = s.kernel_name.startswith("k_torre")
+    names = GROUPS_FUSED if fused else GROUPS_TWOPASS
+    zfrac = (w.cfg.npml[4] + w.cfg.npml[5]) / nz
     groups = {}
     for g in range(5):
         if gcnt[g] == 0:
             continue
         avg = gms[g] / gcnt[g]
         gc = s.group_cells(g)
-        byts = gc * (WORDS_PER_HALF * esz + (1 if g in (2, 3) else 0))
-        groups[GROUPS[g]] = dict(avg_ms=avg, launches=int(gcnt[g]), cells=int(gc),
-                                 gbs=(byts / (avg * 1e-3) / 1e9) if (gc and avg > 0) else None)
+        if fused:
+            byts = gc * (WORDS_FUSED * esz + 1) + (gc * zfrac * PSI_BYTES_Z if g == 0 else 0)
+        else:
+            byts = gc * (WORDS_PER_HALF * esz + (1 if g in (2, 3) else 0))
+        groups[names[g]] = dict(avg_ms=avg, launches=int(gcnt[g]), cells=int(gc),
+                                algorithmic_bytes=int(byts),
+                                gbs=(byts / (avg * 1e-3) / 1e9) if (gc and avg > 0) else None)
     dom = max((g for g in groups if groups[g]["cells"]), key=lambda g: groups[g]["avg_ms"] * groups[g]["launches"])
+    bpc = (WORDS_FUSED * esz + 1) if fused else (2 * WORDS_PER_HALF * esz + 1)
     res = dict(gcells=val, ms=ms, wall_s=wall, groups=groups, dominant=dom, kernel=s.kernel_name,
-               bytes_per_cell_update=2 * WORDS_PER_HALF * esz + 1)
+               bytes_per_cell_update=bpc, fused=fused)
     s.close()
     return res
 
@@ -104,6 +120,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--size", type=str, default="512x256x256")
+    ap.add_argument("--sweep", choices=("twopass", "fused"), default="twopass",
+                    help="time-step schedule of the headline value; the other one is reported beside it")
+    ap.add_argument("--no-alt", action="store_true", help="skip the other schedule")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the benchmark, run a plain device copy of six grid-sized FP32 arrays "
+                         "(the scale reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -111,11 +133,16 @@ def main():
         sys.exit(2)
     nx, ny, nz = [int(v) for v in args.size.split("x")]
     from dtmaxwell4_amd import abi
-    r32 = gpu_run(abi.STORE_FP32, args.steps, args.warmup, nx, ny, nz)
+    sw = abi.SWEEP_FUSED if args.sweep == "fused" else abi.SWEEP_TWOPASS
+    r32 = gpu_run(abi.STORE_FP32, sw, args.steps, args.warmup, nx, ny, nz)
     log(f"[bench] fp32 {r32['gcells']:.2f} Gcells/s  groups={json.dumps(r32['groups'])}")
+    ralt = None
+    if not args.no_alt:
+        ralt = gpu_run(abi.STORE_FP32, 1 - sw, args.steps, args.warmup, nx, ny, nz)
+        log(f"[bench] fp32 other schedule {ralt['gcells']:.2f} Gcells/s  groups={json.dumps(ralt['groups'])}")
     r16 = None
     if not args.no_fp16:
-        r16 = gpu_run(abi.STORE_FP16, args.steps, args.warmup, nx, ny, nz)
+        r16 = gpu_run(abi.STORE_FP16, sw, args.steps, args.warmup, nx, ny, nz)
         log(f"[bench] fp16 {r16['gcells']:.2f} Gcells/s  groups={json.dumps(r16['groups'])}")
     cpu = None
     if not args.no_cpu:
@@ -144,18 +171,35 @@ def main():
                    "grid": [nx, ny, nz], "storage": "fp32", "kernel": r32["kernel"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": r32["dominant"], "avg_launch_ms": dom["avg_ms"],
-                     "algorithmic_bytes_per_cell": WORDS_PER_HALF * 4 + (1 if "E" in r32["dominant"] else 0)},
+                     "kernel": r32["kernel"] + " " + r32["dominant"], "avg_launch_ms": dom["avg_ms"],
+                     "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "cells_per_launch": dom["cells"]},
         "effective_yee_gbs": round(r32["gcells"] * 72, 1),
         "effective_yee_frac": round(r32["gcells"] * 72 / HBM_PEAK_GBS, 4),
         "fp16_store": None if r16 is None else {
             "value": round(r16["gcells"], 3), "unit": "Gcells/s", "speedup_vs_fp32": round(r16["gcells"] / r32["gcells"], 3),
             "dominant_kernel_gbs": r16["groups"][r16["dominant"]]["gbs"]},
+        "other_schedule_fp32": None if ralt is None else {
+            "kernel": ralt["kernel"], "value": round(ralt["gcells"], 3), "unit": "Gcells/s",
+            "groups": ralt["groups"]},
         "adjoint_forward_ratio": None,
         "cpu_baseline": cpu,
         "kernel_groups_fp32": r32["groups"],
     }
     print(json.dumps(out), flush=True)
+    if args.copy_calibration:
+        copy_calibration(nx * ny * nz * 6)
+
+
+def copy_calibration(n):
+    """Three device-to-device copies of n FP32 values: a kernel whose HBM
+    traffic is known exactly (4n bytes read, 4n written per launch)."""
+    import torch
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {3} launches, read {4 * n} B and write {4 * n} B per launch")
 
 
 if __name__ == "__main__":

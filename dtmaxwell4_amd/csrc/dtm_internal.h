@@ -114,6 +114,16 @@ struct Region {  // a box of cells [i0,i1) x [j0,j1) x [k0,k1)
   int i0, i1, j0, j1, k0, k1;
 };
 
+// Regions of the fused torre kernel: per region, tiles of 61 (z) x 13 (y)
+// owned cells and x segments of `seg` planes.
+struct TorreSet {
+  int n;
+  Region r[6];
+  int nbk[6], nbj[6], nbs[6];
+  int start[7];
+  int total, per, seg;
+};
+
 // TF/SF face bands: for each axis a, up to two boxes of cells whose stencil
 // along a can cross a face of the box (the other axes span the box).
 struct TfsfBands {
@@ -154,12 +164,19 @@ void launch_update_e(int store, const Grid& g, const void* fields, const Scales&
 // TF/SF corrections applied after a sweep (half = 0: H, 1: E).  The update is
 // linear in the stencil sums, so the correction is additive, including its
 // CPML psi and ADE J terms.
+// mode 0: H post-correction, 1: E post-correction, 2: H pre-correction
+// (applied to H^{n-1/2} and psi before a fused step: psi += c D / b).
 void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                  const Materials& mt, const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double t_inc,
                  hipStream_t st);
-// Dispersive / gyrotropic cells over the ADE bounding box (after the E sweep).
-void launch_ade(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt,
-                const Ade& ade, hipStream_t st);
+// Dispersive / gyrotropic cells over the ADE bounding box (after the E
+// sweep): E^n read from fields_src, E^{n+1} written to fields_dst.
+void launch_ade(int store, const Grid& g, const void* fields_src, const void* fields_dst, const Scales& sc,
+                const Materials& mt, const Ade& ade, hipStream_t st);
+// Fused time step (dtm_torre.hip): old fields/psi -> new fields/psi.
+void launch_torre(int store, const Grid& g, const void* fields_old, const void* fields_new, const Scales& sc,
+                  const Cpml& pm_old, const Cpml& pm_new, const Materials& mt, const Ade& ade, const Region* regs,
+                  int nreg, const Region& main, hipStream_t st);
 void launch_soft(int store, const Grid& g, const void* fields, const Scales& sc, const Soft& so,
                  hipStream_t st);
 void launch_record(int store, const Grid& g, const void* fields, const Scales& sc, const Probes& pr,

@@ -390,7 +390,8 @@ __global__ __launch_bounds__(BZ* MBY) void k_update_h(Grid g, Fields<STORE> f, S
 // Dispersive / gyrotropic cells: countDisp (dispersion.cpp:85-99) and the
 // magnetised-plasma current, from the curl H the E sweep stored.
 template <int STORE>
-__global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Scales sc, Materials mt, Ade ade) {
+__global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Fields<STORE> fd, Scales sc, Materials mt,
+                                             Ade ade) {
   const long long nb = (long long)ade.ni * ade.nj * ade.nk;
   for (long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x; b < nb; b += (long long)gridDim.x * blockDim.x) {
     const int k = ade.k0 + (int)(b % ade.nk);
@@ -433,9 +434,9 @@ __global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Scales sc,
           ade.Jp[c][p][b] = jn;
         }
         ade.Em[c][b] = Ep;
-        st(f.E[c], q, Enew, sc.se[c]);
+        st(fd.E[c], q, Enew, sc.se[c]);
       } else {
-        st(f.E[c], q, En[c] + g.dt * ce.inv_eps[c] * curl[c], sc.se[c]);
+        st(fd.E[c], q, En[c] + g.dt * ce.inv_eps[c] * curl[c], sc.se[c]);
       }
     }
   }
@@ -450,9 +451,13 @@ __global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Scales sc,
 // with s = +1 for the first curl term and -1 for the second; a dispersive
 // component gets C3 * s * (1+c_pml) * D and its J^{n+1} the matching
 // kJ2 * dE (dispersion.cpp:84-99 is linear in the curl).
-template <int STORE, int HALF>
+// MODE 2 (H pre-correction before a fused step, on H^{n-1/2}): the fused
+// step's own psi update then carries c*D, so psi gets c*D/b and H gets
+// -dt*s*D only.
+template <int STORE, int MODE>
 __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt, Ade ade,
                                               Tfsf tf, TfsfBands bands, double tinc) {
+  constexpr int HALF = MODE == 1 ? 1 : 0;
   long long acc = 0;
   const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   int r = 0;
@@ -492,7 +497,12 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
         cp = HALF == 0 ? pm.cH[a][vpos[a]] : pm.cE[a][vpos[a]];
         float* psi = HALF == 0 ? pm.psiH[a][slot] : pm.psiE[a][slot];
         const long long pq = psi_index(g, pm, a, si, i, j, k);
-        psi[pq] += cp * D;
+        if (MODE == 2) {
+          psi[pq] += cp * D / pm.bH[a][vpos[a]];
+          cp = 0.0f;
+        } else {
+          psi[pq] += cp * D;
+        }
       }
     }
     const float dcurl = sgn * (1.0f + cp) * D;
@@ -824,7 +834,7 @@ void launch_update_e(int store, const Grid& g, const void* fields, const Scales&
 // One launch per face-normal axis: bands of different axes overlap at the
 // box edges and both update the shared component there (e.g. Hz from the x
 // and y faces), so they must not run concurrently.
-template <int STORE, int HALF>
+template <int STORE, int MODE>
 static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                     const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double tinc, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
@@ -840,7 +850,7 @@ static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const C
       n += (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
     }
     if (n <= 0) continue;
-    hipLaunchKernelGGL((k_tfsf<STORE, HALF>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, f, sc, pm, mt,
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, f, sc, pm, mt,
                        ade, tf, sub, tinc);
     check_launch("k_tfsf");
   }
@@ -851,21 +861,23 @@ void launch_tfsf(int store, int half, const Grid& g, const void* fields, const S
                  hipStream_t st) {
   if (store == 0 && half == 0) tfsf_go<0, 0>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
   if (store == 0 && half == 1) tfsf_go<0, 1>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
+  if (store == 0 && half == 2) tfsf_go<0, 2>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
   if (store == 1 && half == 0) tfsf_go<1, 0>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
   if (store == 1 && half == 1) tfsf_go<1, 1>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
+  if (store == 1 && half == 2) tfsf_go<1, 2>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
 }
 
-void launch_ade(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt, const Ade& ade,
-                hipStream_t st) {
+void launch_ade(int store, const Grid& g, const void* fsrc, const void* fdst, const Scales& sc, const Materials& mt,
+                const Ade& ade, hipStream_t st) {
   if (!ade.on) return;
   const long long nb = (long long)ade.ni * ade.nj * ade.nk;
   const unsigned blocks = (unsigned)std::min<long long>((nb + 255) / 256, 65536);
   if (store == 0)
-    hipLaunchKernelGGL(k_ade<0>, dim3(blocks), dim3(256), 0, st, g, *static_cast<const Fields<0>*>(fields), sc, mt,
-                       ade);
+    hipLaunchKernelGGL(k_ade<0>, dim3(blocks), dim3(256), 0, st, g, *static_cast<const Fields<0>*>(fsrc),
+                       *static_cast<const Fields<0>*>(fdst), sc, mt, ade);
   else
-    hipLaunchKernelGGL(k_ade<1>, dim3(blocks), dim3(256), 0, st, g, *static_cast<const Fields<1>*>(fields), sc, mt,
-                       ade);
+    hipLaunchKernelGGL(k_ade<1>, dim3(blocks), dim3(256), 0, st, g, *static_cast<const Fields<1>*>(fsrc),
+                       *static_cast<const Fields<1>*>(fdst), sc, mt, ade);
   check_launch("k_ade");
 }
 

@@ -45,11 +45,15 @@ struct dtm_solver {
   Grid g{};
   Scales sc{};
   int store = 0;
-  void* E[3] = {};
+  void* E[3] = {};   // current fields (E^n, H^{n-1/2} before a step)
   void* H[3] = {};
-  Fields<0> f32{};
-  Fields<1> f16{};
-  Cpml pm{};
+  void* E2[3] = {};  // second buffers of the fused (torre) step
+  void* H2[3] = {};
+  Fields<0> f32{}, f32n{};
+  Fields<1> f16{}, f16n{};
+  Cpml pm{};         // current psi
+  Cpml pmn{};        // second psi buffers (torre step)
+  bool torre = false; // fused single-pass step; false: two sweeps per step
   std::vector<void*> allocs;  // everything to free
   Region main{};
   Region edge[4]{};
@@ -425,17 +429,32 @@ static int setup_cpml(dtm_solver* s) {
       if (int rc = dalloc(s, &p.psiH[a][sl], sizeof(float) * cnt)) return rc;
     }
   }
+  s->pmn = p;
+  if (s->torre) {
+    for (int a = 0; a < 3; a++) {
+      const int P = p.plo[a] + p.phi[a];
+      size_t cnt = 0;
+      if (P > 0) cnt = (size_t)P * (a == 0 ? (size_t)c.ny * c.nz : a == 1 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny);
+      for (int sl = 0; sl < 2; sl++) {
+        if (int rc = dalloc(s, &s->pmn.psiE[a][sl], sizeof(float) * cnt)) return rc;
+        if (int rc = dalloc(s, &s->pmn.psiH[a][sl], sizeof(float) * cnt)) return rc;
+      }
+    }
+  }
   return 0;
 }
 
 // Interior box: every x/y stencil neighbour exists and no x/y CPML; edges
-// take the rest with full boundary handling.
+// take the rest with full boundary handling.  The fused step recomputes H
+// one cell below and two cells above the E box it owns (x planes and y rows)
+// and reads E three cells beyond it, so the interior stops one cell short of
+// the low CPML slab and two short of the high one: its recomputed H then
+// never needs the x/y CPML terms the interior kernel leaves out.
 static void setup_regions(dtm_solver* s) {
   const dtm_config& c = s->cfg;
   auto edge = [&](int a, int lo_or_hi) {
-    const int p = c.npml[2 * a + lo_or_hi];
-    const int per = c.bc[a] == DTM_BC_PERIODIC;
-    return std::max(p, per || true ? 2 : 0);
+    const int n = c.npml[2 * a + lo_or_hi];
+    return std::max(n > 0 ? n + 1 + lo_or_hi : 0, 3);
   };
   int x0 = std::min(edge(0, 0), c.nx), x1 = std::max(c.nx - edge(0, 1), x0);
   int y0 = std::min(edge(1, 0), c.ny), y1 = std::max(c.ny - edge(1, 1), y0);
@@ -453,11 +472,32 @@ static void field_structs(dtm_solver* s) {
     s->f32.H[c] = static_cast<float*>(s->H[c]);
     s->f16.E[c] = static_cast<__half*>(s->E[c]);
     s->f16.H[c] = static_cast<__half*>(s->H[c]);
+    s->f32n.E[c] = static_cast<float*>(s->E2[c]);
+    s->f32n.H[c] = static_cast<float*>(s->H2[c]);
+    s->f16n.E[c] = static_cast<__half*>(s->E2[c]);
+    s->f16n.H[c] = static_cast<__half*>(s->H2[c]);
   }
 }
 
 static const void* fields_ptr(const dtm_solver* s) {
   return s->store == 0 ? static_cast<const void*>(&s->f32) : static_cast<const void*>(&s->f16);
+}
+static const void* fields_next_ptr(const dtm_solver* s) {
+  return s->store == 0 ? static_cast<const void*>(&s->f32n) : static_cast<const void*>(&s->f16n);
+}
+
+// After a fused step the new buffers become current.
+static void swap_buffers(dtm_solver* s) {
+  for (int c = 0; c < 3; c++) {
+    std::swap(s->E[c], s->E2[c]);
+    std::swap(s->H[c], s->H2[c]);
+  }
+  field_structs(s);
+  for (int a = 0; a < 3; a++)
+    for (int sl = 0; sl < 2; sl++) {
+      std::swap(s->pm.psiE[a][sl], s->pmn.psiE[a][sl]);
+      std::swap(s->pm.psiH[a][sl], s->pmn.psiH[a][sl]);
+    }
 }
 
 // Build the class table from per-component material ids.
@@ -584,6 +624,7 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   if (c.nmat < 1 || c.nmat > DTM_MAX_MATS) return fail(DTM_ERR_ARG, "nmat out of range");
   if (c.nsoft < 0 || c.nsoft > DTM_MAX_SOFT) return fail(DTM_ERR_ARG, "nsoft out of range");
   if (c.storage != DTM_STORE_FP32 && c.storage != DTM_STORE_FP16) return fail(DTM_ERR_ARG, "bad storage");
+  if (c.sweep != DTM_SWEEP_TWOPASS && c.sweep != DTM_SWEEP_FUSED) return fail(DTM_ERR_ARG, "bad sweep");
   const int n3[3] = {c.nx, c.ny, c.nz};
   for (int a = 0; a < 3; a++) {
     if (c.npml[2 * a] < 0 || c.npml[2 * a + 1] < 0 || c.npml[2 * a] + c.npml[2 * a + 1] > n3[a])
@@ -605,6 +646,7 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   dtm_solver* s = new dtm_solver();
   s->cfg = c;
   s->store = c.storage;
+  s->torre = c.sweep == DTM_SWEEP_FUSED;
   hipError_t e = hipGetDevice(&s->dev);
   if (e != hipSuccess) {
     delete s;
@@ -639,6 +681,8 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   for (int q = 0; q < 3 && !rc; q++) {
     rc = dalloc(s, &s->E[q], esz * n);
     if (!rc) rc = dalloc(s, &s->H[q], esz * n);
+    if (!rc && s->torre) rc = dalloc(s, &s->E2[q], esz * n);
+    if (!rc && s->torre) rc = dalloc(s, &s->H2[q], esz * n);
   }
   if (!rc) rc = setup_cpml(s);
   if (!rc) rc = setup_tfsf(s);
@@ -807,28 +851,50 @@ static int one_step(dtm_solver* s) {
   s->tf.active = tfsf_gate(s, n) ? 1 : 0;
   const void* F = fields_ptr(s);
   const double th = (double)n * c.dt, te = ((double)n + 0.5) * c.dt;
-  // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
-  ev_begin(s, 0);
-  launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main, s->stream);
-  ev_end(s);
-  ev_begin(s, 1);
-  launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge, s->nedge, kEmpty, s->stream);
-  ev_end(s);
-  if (s->tf.active) {
-    ev_begin(s, 4);
-    launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, th, s->stream);
+  if (s->torre) {
+    // TF/SF on H^{n-1/2} before the fused step (exact: the step is linear)
+    if (s->tf.active) {
+      ev_begin(s, 4);
+      launch_tfsf(s->store, 2, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, th, s->stream);
+      ev_end(s);
+    }
+    ev_begin(s, 0);
+    launch_torre(s->store, s->g, F, fields_next_ptr(s), s->sc, s->pm, s->pmn, s->mt, s->ade, nullptr, 0, s->main,
+                 s->stream);
     ev_end(s);
+    ev_begin(s, 1);
+    launch_torre(s->store, s->g, F, fields_next_ptr(s), s->sc, s->pm, s->pmn, s->mt, s->ade, s->edge, s->nedge, kEmpty,
+                 s->stream);
+    ev_end(s);
+    ev_begin(s, 4);
+    launch_ade(s->store, s->g, F, fields_next_ptr(s), s->sc, s->mt, s->ade, s->stream);
+    swap_buffers(s);
+    F = fields_ptr(s);
+    if (s->tf.active) launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, te, s->stream);
+  } else {
+    // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
+    ev_begin(s, 0);
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main, s->stream);
+    ev_end(s);
+    ev_begin(s, 1);
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge, s->nedge, kEmpty, s->stream);
+    ev_end(s);
+    if (s->tf.active) {
+      ev_begin(s, 4);
+      launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, th, s->stream);
+      ev_end(s);
+    }
+    // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
+    ev_begin(s, 2);
+    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main, s->stream);
+    ev_end(s);
+    ev_begin(s, 3);
+    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge, s->nedge, kEmpty, s->stream);
+    ev_end(s);
+    ev_begin(s, 4);
+    launch_ade(s->store, s->g, F, F, s->sc, s->mt, s->ade, s->stream);
+    if (s->tf.active) launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, te, s->stream);
   }
-  // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
-  ev_begin(s, 2);
-  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main, s->stream);
-  ev_end(s);
-  ev_begin(s, 3);
-  launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge, s->nedge, kEmpty, s->stream);
-  ev_end(s);
-  ev_begin(s, 4);
-  launch_ade(s->store, s->g, F, s->sc, s->mt, s->ade, s->stream);
-  if (s->tf.active) launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, te, s->stream);
   if (c.nsoft > 0) {
     Soft so{};
     so.n = c.nsoft;
@@ -982,7 +1048,10 @@ int dtm_time_steps(dtm_solver* s, int nsteps, double* ms) {
   return 0;
 }
 
-const char* dtm_kernel_name(const dtm_solver* s) { return kernel_variant_name(s ? s->store : 0); }
+const char* dtm_kernel_name(const dtm_solver* s) {
+  if (s && s->torre) return s->store == 0 ? "k_torre<fp32>" : "k_torre<fp16>";
+  return kernel_variant_name(s ? s->store : 0);
+}
 
 int dtm_set_kernel_timing(dtm_solver* s, int on) {
   if (!s) return fail(DTM_ERR_ARG, "null solver");

@@ -144,6 +144,12 @@ typedef struct dtm_soft_source {
   double amp, w, t0, tw, kEnv;
 } dtm_soft_source;
 
+/* Time-step schedule.  TWOPASS: one sweep for H, one for E (72 B per
+ * cell-update in FP32).  FUSED: H and E of one step in a single x-marching
+ * sweep over double-buffered fields (48 B per cell-update plus halo
+ * recompute); needs a second copy of the fields. */
+enum { DTM_SWEEP_TWOPASS = 0, DTM_SWEEP_FUSED = 1 };
+
 typedef struct dtm_config {
   int nx, ny, nz;
   double dx, dy, dz, dt;
@@ -163,6 +169,7 @@ typedef struct dtm_config {
                              granularity and legacy sensor-write cadence   */
   int nsoft;
   dtm_soft_source soft[DTM_MAX_SOFT];
+  int sweep;              /* DTM_SWEEP_*: time-step schedule (same results) */
 } dtm_config;
 
 /* Fill a config with the reference's defaults (dx=dy=dz=0.005, dt=0.002,

@@ -12,12 +12,23 @@ from oracle.oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
+SWEEP = abi.SWEEP_TWOPASS
+
+
+@pytest.fixture(autouse=True, params=[abi.SWEEP_TWOPASS, abi.SWEEP_FUSED], ids=["twopass", "fused"])
+def sweep(request):
+    """Every parity case runs under both time-step schedules."""
+    global SWEEP
+    SWEEP = request.param
+    yield request.param
+    SWEEP = abi.SWEEP_TWOPASS
 
 
 def solver(cfg):
     from dtmaxwell4_amd.solver import Solver
     lib = abi.load_library()
     assert lib.dtm_device_count() > 0, "no HIP device visible"
+    cfg.sweep = SWEEP
     return Solver(cfg)
 
 
@@ -169,6 +180,25 @@ def test_zero_boundary_no_pml_and_lossy_material():
     so.waveform, so.amp, so.w, so.tw, so.t0 = abi.WAVE_GAUSS_SINE, 1.0, 2 * np.pi, 0.3, 1.0
     mats = configs.sphere_mats(cfg_shape := (cfg.nx, cfg.ny, cfg.nz), (d, d, d), (0.8, 0.8, 0.6), 0.25, 1)
     w = configs.Workload("zero", cfg, mats=mats, steps=300)
+    g, o = run_pair(w)
+    check_fields(g, o)
+
+
+@pytest.mark.parametrize("npml", [(6, 6, 6, 6, 6, 6), (4, 9, 7, 3, 5, 8)])
+def test_cpml_next_to_interior_boundary(npml):
+    """A source close to a corner keeps the fields strong in the first CPML
+    cells beside the interior box for hundreds of steps: the interior and
+    edge launches must agree on every H value they both compute there."""
+    nx, ny, nz, d = 40, 36, 32, 0.05
+    cfg = abi.default_config(nx, ny, nz, d, d, d, configs.dt_for(d, d, d))
+    cfg.npml[:] = list(npml)
+    cfg.nsoft = 2
+    for q, (comp, i, j, k) in enumerate([(abi.EZ, npml[0] + 3, ny - npml[3] - 4, nz // 2),
+                                         (abi.EX, nx - npml[1] - 3, npml[2] + 3, npml[4] + 3)]):
+        so = cfg.soft[q]
+        so.comp, so.i, so.j, so.k = comp, i, j, k
+        so.waveform, so.amp, so.w, so.tw, so.t0 = abi.WAVE_GAUSS_SINE, 1.0, 2 * np.pi / 0.4, 0.15, 0.5
+    w = configs.Workload("cpml_corner", cfg, steps=400)
     g, o = run_pair(w)
     check_fields(g, o)
 
