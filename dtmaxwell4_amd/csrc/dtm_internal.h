@@ -22,6 +22,7 @@ struct Grid {
   int per[3];        // periodic axes
   float rd[3];       // 1/(24 d_a): the (2,4) weights 27/24, -1/24 (defs.h:3-13)
   float dt;
+  const void* zero;  // 256 zero bytes: the load target of positions outside the grid
 };
 
 // CPML: psi lives only in the slabs.  For axis a the two affected
@@ -94,6 +95,13 @@ struct Tfsf {
   double st, ct, sph, cph, sps, cps;  // plane wave angles
   double kx, ky, kz, pos[3];
   double dip[3];  // dipole position
+  // plane wave with k along one axis: the incident component c at
+  // half-coordinate h is P[c] * table[h_a - tab_h0], with table the
+  // waveform amp*f0(phase) of the current half-step along axis tab_axis
+  int tab_axis;  // -1: evaluate per point
+  int tab_h0, tab_n;
+  const double* table;
+  double P[6];
 };
 
 template <int STORE> struct Fields;
@@ -166,6 +174,14 @@ void launch_update_e(int store, const Grid& g, const void* fields, const Scales&
 // CPML psi and ADE J terms.
 // mode 0: H post-correction, 1: E post-correction, 2: H pre-correction
 // (applied to H^{n-1/2} and psi before a fused step: psi += c D / b).
+// design-gradient adjoint (FP32 storage)
+void launch_adj_snap(const Grid& g, const void* fields, const Materials& mt, float* out, hipStream_t st);
+void launch_adj_monitor(const void* fields, const long long* q, int nmon, float* series, hipStream_t st);
+void launch_adj_grad(const Grid& g, const void* fields, const Materials& mt, const float* e0, const float* e1,
+                     double* acc, hipStream_t st);
+void launch_adj_inject(const void* fields, const long long* q, const float* inv_eps, const double* F, int nmon,
+                       double w, double dt, long long n, hipStream_t st);
+void launch_tfsf_tables(const Tfsf& tf, double* out, long long n0, int nsteps, double dt, hipStream_t st);
 void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                  const Materials& mt, const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double t_inc,
                  hipStream_t st);

@@ -148,6 +148,9 @@ __device__ __forceinline__ int comp_at(int hx, int hy, int hz) {
 
 // Sum over the 4 stencil neighbours along axis a of component-at-(hp) of the
 // TF/SF corrections (genAlg.py:219-221) weighted by (+1,-27,+27,-1).
+// TAB: plane wave with k along one axis, the incident value is the
+// polarisation factor P[comp] times this half-step's waveform table entry.
+template <bool TAB>
 __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t) {
   const bool psf = in_sf(s, hx, hy, hz);
   const int ho[4] = {-3, -1, 1, 3};
@@ -160,7 +163,13 @@ __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int h
     else h2 += ho[q];
     const bool qsf = in_sf(s, h0, h1, h2);
     if (qsf == psf) continue;
-    const double v = incident(s, comp_at(h0, h1, h2), h0, h1, h2, t);
+    double v;
+    if (TAB) {
+      const int ha = s.tab_axis == 0 ? h0 : (s.tab_axis == 1 ? h1 : h2);
+      v = s.P[comp_at(h0, h1, h2)] * s.table[min(max(ha - s.tab_h0, 0), s.tab_n - 1)];
+    } else {
+      v = incident(s, comp_at(h0, h1, h2), h0, h1, h2, t);
+    }
     acc += wt[q] * ((qsf && !psf) ? v : -v);
   }
   return (float)acc;
@@ -184,6 +193,12 @@ __device__ __forceinline__ long long psi_index(const Grid& g, const Cpml& p, int
 // psi <- b psi + c D ; D <- D + psi   (CPML, kappa=1, alpha=0)
 __device__ __forceinline__ void cpml_apply(float* psi, long long pq, float b, float c, float& D) {
   float v = b * psi[pq] + c * D;
+  psi[pq] = v;
+  D += v;
+}
+// The same with the old psi value already loaded.
+__device__ __forceinline__ void cpml_step(float* psi, long long pq, float old, float b, float c, float& D) {
+  const float v = b * old + c * D;
   psi[pq] = v;
   D += v;
 }
@@ -274,12 +289,53 @@ __device__ __forceinline__ float ldp(const T* F, long long base, long long off, 
 // compiler from moving LDS accesses across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Raw storage element and its conversion: prefetched operands stay in the
+// storage type until they are published to LDS, so no conversion (and no
+// wait for the load) sits next to the load that produced them.
+template <int STORE> struct Raw;
+template <> struct Raw<0> { using T = float; };
+template <> struct Raw<1> { using T = __half; };
+__device__ __forceinline__ float cvt(float v, float) { return v; }
+__device__ __forceinline__ float cvt(__half v, float inv) { return __half2float(v) * inv; }
+template <typename T> __device__ __forceinline__ T rzero();
+template <> __device__ __forceinline__ float rzero<float>() { return 0.0f; }
+template <> __device__ __forceinline__ __half rzero<__half>() { return __ushort_as_half((unsigned short)0); }
+template <typename T>
+__device__ __forceinline__ T ldr(const T* F, long long base, long long off) {
+  return (base < 0 || off < 0) ? rzero<T>() : F[base + off];
+}
+// Address of a position under the boundary rule: outside the grid (base or
+// off < 0) it is the zero page, so the load itself is unconditional and its
+// destination register never needs a merge (a merge would force a wait for
+// the load before the next plane).
+template <typename T>
+__device__ __forceinline__ const T* at(const T* F, long long base, long long off, const void* zero) {
+  return (base < 0 || off < 0) ? static_cast<const T*>(zero) : F + base + off;
+}
+
+// Ring slot of plane p in a 5-plane LDS ring (p >= -3).
+__device__ __forceinline__ int ring5(int p) { return (p + 10) % 5; }
+
 // ================================================================ H sweep
-template <int STORE, bool GEN>
+// Operands of one plane, loaded two planes ahead: the Ex tile values of
+// plane p (own, y-halo row, z-halo column), the H read-modify-write
+// operands of plane p, and the Ey / Ez tile values of plane p+1 (the x
+// stencil of H reaches one plane ahead).
+template <typename T> struct PreH {
+  T ex, exy, exz, hx, hy, hz, ey, eyz, ez, ezy;
+};
+
+// GEN bit 0: x boundary rule and x CPML (per plane, block-uniform);
+// bit 1: y CPML (per row).  The y and z boundary rules and the z CPML are
+// always on: they are per-thread constants.
+template <int STORE, int GEN>
 __global__ __launch_bounds__(BZ* MBY) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
-  __shared__ float sEx[2][MBY + 3][MLZ];  // [row j-j0+2][col k-k0+2]
-  __shared__ float sEy[2][MBY][MLZ];      // [row j-j0][col k-k0+2]
-  __shared__ float sEz[2][MBY + 3][BZ];   // [row j-j0+2][col k-k0]
+  using T = typename Raw<STORE>::T;
+  constexpr bool GX = GEN & 1, GY = GEN & 2;
+  __shared__ float sEx[2][MBY + 3][MLZ];  // [row j-j0+2][col k-k0+2], plane i
+  __shared__ float rEy[5][MBY][MLZ];      // [row j-j0][col k-k0+2], ring of planes
+  __shared__ float rEz[5][MBY + 3][BZ];   // [row j-j0+2][col k-k0], ring of planes
+  __shared__ float sHo[3][MBY][BZ];       // this thread's H operands of plane i
   MarchTile t;
   if (!march_locate(ms, t)) return;
   const Region& R = ms.r[t.r];
@@ -287,103 +343,144 @@ __global__ __launch_bounds__(BZ* MBY) void k_update_h(Grid g, Fields<STORE> f, S
   const int j = t.j0 + ty, k = t.k0 + tx;
   const bool inreg = j < R.j1 && k < R.k1;
   const long long own = plane_off(g, j, k);
-  // halo positions: rows j0-2, j0-1, j0+MBY (threads ty<3); columns k0-2, k0-1, k0+64 (tx<3)
+  // halo positions: rows j0-2, j0-1, j0+MBY (waves ty<3); columns k0-2, k0-1, k0+64 (lanes tx<3)
+  const bool yw = ty < 3, zl = tx < 3;
   const int yrow = ty < 2 ? ty : MBY + 2;
-  const long long yh = ty < 3 ? plane_off(g, t.j0 + (ty < 2 ? ty - 2 : MBY), k) : -1;
+  const long long yh = yw ? plane_off(g, t.j0 + (ty < 2 ? ty - 2 : MBY), k) : -1;
   const int zcol = tx < 2 ? tx : BZ + 2;
-  const long long zh = tx < 3 ? plane_off(g, j, t.k0 + (tx < 2 ? tx - 2 : BZ)) : -1;
-  const auto* Ex = f.E[0];
-  const auto* Ey = f.E[1];
-  const auto* Ez = f.E[2];
-  auto* Hx = f.H[0];
-  auto* Hy = f.H[1];
-  auto* Hz = f.H[2];
+  const long long zh = zl ? plane_off(g, j, t.k0 + (tx < 2 ? tx - 2 : BZ)) : -1;
+  const T* Ex = f.E[0];
+  const T* Ey = f.E[1];
+  const T* Ez = f.E[2];
+  T* Hx = f.H[0];
+  T* Hy = f.H[1];
+  T* Hz = f.H[2];
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
-  // x queue: Ey, Ez at planes i-2 .. i+1
-  float qy[4], qz[4];
-  for (int s = 0; s < 4; s++) {
-    const long long pb = plane_base<GEN>(g, t.i0 - 2 + s);
-    qy[s] = ldp(Ey, pb, own, iey);
-    qz[s] = ldp(Ez, pb, own, iez);
-  }
+  const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
+  const T z0 = rzero<T>();
+
+  const long long hq = inreg ? own : -1;  // H operands: owned cells only
+  auto fetch = [&](PreH<T>& S, int p) {
+    const long long pb = plane_base<GX>(g, p);
+    const long long pn = plane_base<GX>(g, p + 1);
+    S.ex = *at(Ex, pb, own, g.zero);
+    S.ey = *at(Ey, pn, own, g.zero);
+    S.ez = *at(Ez, pn, own, g.zero);
+    S.exy = *at(Ex, pb, yh, g.zero);
+    S.ezy = *at(Ez, pn, yh, g.zero);
+    S.exz = *at(Ex, pb, zh, g.zero);
+    S.eyz = *at(Ey, pn, zh, g.zero);
+    S.hx = *at((const T*)Hx, pb, hq, g.zero);
+    S.hy = *at((const T*)Hy, pb, hq, g.zero);
+    S.hz = *at((const T*)Hz, pb, hq, g.zero);
+  };
+  // Ey / Ez tile of plane p straight into its ring slot (prologue only)
+  auto ring_direct = [&](int p) {
+    const long long pb = plane_base<GX>(g, p);
+    const int s = ring5(p);
+    rEy[s][ty][tx + 2] = cvt(ldr(Ey, pb, own), iey);
+    rEz[s][ty + 2][tx] = cvt(ldr(Ez, pb, own), iez);
+    if (yw) rEz[s][yrow][tx] = cvt(ldr(Ez, pb, yh), iez);
+    if (zl) rEy[s][ty][zcol] = cvt(ldr(Ey, pb, zh), iey);
+  };
+
   const int zs = pm.on ? slab_index(k, g.nz, pm.plo[2], pm.phi[2]) : -1;
-  const int ysl = (GEN && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
+  const int ysl = (GY && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
   const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
-  // plane-i operands, loaded one plane ahead: own Ex, the two halo values
-  // and the H read-modify-write operands
-  float n_ex, n_y0 = 0.f, n_y1 = 0.f, n_z0 = 0.f, n_z1 = 0.f, n_hx = 0.f, n_hy = 0.f, n_hz = 0.f;
-  {
-    const long long pb = plane_base<GEN>(g, t.i0);
-    n_ex = ldp(Ex, pb, own, iex);
-    if (ty < 3) { n_y0 = ldp(Ex, pb, yh, iex); n_y1 = ldp(Ez, pb, yh, iez); }
-    if (tx < 3) { n_z0 = ldp(Ex, pb, zh, iex); n_z1 = ldp(Ey, pb, zh, iey); }
-    if (inreg) { n_hx = ld(Hx, pb + own, sc.ih[0]); n_hy = ld(Hy, pb + own, sc.ih[1]); n_hz = ld(Hz, pb + own, sc.ih[2]); }
-  }
-  for (int i = t.i0; i < t.i1; i++) {
+  // CPML coefficients of this column (z and y are fixed per thread)
+  const float bz = zs >= 0 ? pm.bH[2][k] : 0.f, cz = zs >= 0 ? pm.cH[2][k] : 0.f;
+  const float by = ysl >= 0 ? pm.bH[1][j] : 0.f, cy = ysl >= 0 ? pm.cH[1][j] : 0.f;
+
+  PreH<T> A{z0, z0, z0, z0, z0, z0, z0, z0, z0, z0}, B = A;
+  ring_direct(t.i0 - 2);
+  ring_direct(t.i0 - 1);
+  ring_direct(t.i0);
+  // Prefetches are unconditional (a plane past the segment is clamped to
+  // its last plane): with no branch around them the compiler's in-order
+  // wait counts stay exact and a wait never reaches the other set.
+  fetch(A, t.i0);
+  fetch(B, min(t.i0 + 1, t.i1 - 1));
+
+  auto step = [&](int i, PreH<T>& S) {
     const int b = i & 1;
-    const long long pb = plane_base<GEN>(g, i);
-    const float exo = n_ex, hx0 = n_hx, hy0 = n_hy, hz0 = n_hz;
-    sEx[b][ty + 2][tx + 2] = exo;
-    sEy[b][ty][tx + 2] = qy[2];
-    sEz[b][ty + 2][tx] = qz[2];
-    if (ty < 3) {
-      sEx[b][yrow][tx + 2] = n_y0;
-      sEz[b][yrow][tx] = n_y1;
+    const int s0 = ring5(i - 2), s1 = ring5(i - 1), s2 = ring5(i), s3 = ring5(i + 1);
+    // publish plane i (Ex) and plane i+1 (Ey, Ez)
+    sEx[b][ty + 2][tx + 2] = cvt(S.ex, iex);
+    rEy[s3][ty][tx + 2] = cvt(S.ey, iey);
+    rEz[s3][ty + 2][tx] = cvt(S.ez, iez);
+    if (yw) {
+      sEx[b][yrow][tx + 2] = cvt(S.exy, iex);
+      rEz[s3][yrow][tx] = cvt(S.ezy, iez);
     }
-    if (tx < 3) {
-      sEx[b][ty + 2][zcol] = n_z0;
-      sEy[b][ty][zcol] = n_z1;
+    if (zl) {
+      sEx[b][ty + 2][zcol] = cvt(S.exz, iex);
+      rEy[s3][ty][zcol] = cvt(S.eyz, iey);
     }
-    // prefetch plane i+1 operands and the queue's plane i+2
-    float nqy = 0.f, nqz = 0.f;
-    if (i + 1 < t.i1) {
-      const long long p1 = plane_base<GEN>(g, i + 1);
-      const long long p2 = plane_base<GEN>(g, i + 2);
-      n_ex = ldp(Ex, p1, own, iex);
-      if (ty < 3) { n_y0 = ldp(Ex, p1, yh, iex); n_y1 = ldp(Ez, p1, yh, iez); }
-      if (tx < 3) { n_z0 = ldp(Ex, p1, zh, iex); n_z1 = ldp(Ey, p1, zh, iey); }
-      if (inreg) { n_hx = ld(Hx, p1 + own, sc.ih[0]); n_hy = ld(Hy, p1 + own, sc.ih[1]); n_hz = ld(Hz, p1 + own, sc.ih[2]); }
-      nqy = ldp(Ey, p2, own, iey);
-      nqz = ldp(Ez, p2, own, iez);
+    // the H operands go through this thread's own LDS slot: no register of
+    // the prefetch set is live across its refetch
+    sHo[0][ty][tx] = cvt(S.hx, ihx);
+    sHo[1][ty][tx] = cvt(S.hy, ihy);
+    sHo[2][ty][tx] = cvt(S.hz, ihz);
+    // CPML psi of plane i, loaded before the prefetch so that waiting for
+    // psi does not wait for the prefetch (loads complete in issue order)
+    float pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f, py0 = 0.f, py1 = 0.f;
+    long long qz = 0, qx = 0, qy = 0;
+    const int xsl = (GX && pm.on) ? slab_index(i, g.nx, pm.plo[0], pm.phi[0]) : -1;
+    if (inreg && zs >= 0) {
+      qz = psi_index(g, pm, 2, zs, i, j, k);
+      pz0 = pm.psiH[2][0][qz];
+      pz1 = pm.psiH[2][1][qz];
     }
+    float bx = 0.f, cx = 0.f;
+    if (GX && xsl >= 0) {
+      bx = pm.bH[0][i];
+      cx = pm.cH[0][i];
+    }
+    if (GX && inreg && xsl >= 0) {
+      qx = psi_index(g, pm, 0, xsl, i, j, k);
+      px0 = pm.psiH[0][0][qx];
+      px1 = pm.psiH[0][1][qx];
+    }
+    if (GY && inreg && ysl >= 0) {
+      qy = psi_index(g, pm, 1, ysl, i, j, k);
+      py0 = pm.psiH[1][0][qy];
+      py1 = pm.psiH[1][1][qy];
+    }
+    fetch(S, min(i + 2, t.i1 - 1));
     lds_barrier();
     if (inreg) {
-      float dyEz = rd1 * d24(sEz[b][ty][tx], sEz[b][ty + 1][tx], qz[2], sEz[b][ty + 3][tx]);
+      const float hx0 = sHo[0][ty][tx], hy0 = sHo[1][ty][tx], hz0 = sHo[2][ty][tx];
+      const float exo = sEx[b][ty + 2][tx + 2];
+      float dyEz = rd1 * d24(rEz[s2][ty][tx], rEz[s2][ty + 1][tx], rEz[s2][ty + 2][tx], rEz[s2][ty + 3][tx]);
       float dyEx = rd1 * d24(sEx[b][ty][tx + 2], sEx[b][ty + 1][tx + 2], exo, sEx[b][ty + 3][tx + 2]);
-      float dzEy = rd2 * d24(sEy[b][ty][tx], sEy[b][ty][tx + 1], qy[2], sEy[b][ty][tx + 3]);
+      float dzEy = rd2 * d24(rEy[s2][ty][tx], rEy[s2][ty][tx + 1], rEy[s2][ty][tx + 2], rEy[s2][ty][tx + 3]);
       float dzEx = rd2 * d24(sEx[b][ty + 2][tx], sEx[b][ty + 2][tx + 1], exo, sEx[b][ty + 2][tx + 3]);
-      float dxEz = rd0 * d24(qz[0], qz[1], qz[2], qz[3]);
-      float dxEy = rd0 * d24(qy[0], qy[1], qy[2], qy[3]);
+      float dxEz = rd0 * d24(rEz[s0][ty + 2][tx], rEz[s1][ty + 2][tx], rEz[s2][ty + 2][tx], rEz[s3][ty + 2][tx]);
+      float dxEy = rd0 * d24(rEy[s0][ty][tx + 2], rEy[s1][ty][tx + 2], rEy[s2][ty][tx + 2], rEy[s3][ty][tx + 2]);
       if (zs >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
-        const float bb = pm.bH[2][k], cc = pm.cH[2][k];
-        const long long pq = psi_index(g, pm, 2, zs, i, j, k);
-        cpml_apply(pm.psiH[2][0], pq, bb, cc, dzEy);
-        cpml_apply(pm.psiH[2][1], pq, bb, cc, dzEx);
+        cpml_step(pm.psiH[2][0], qz, pz0, bz, cz, dzEy);
+        cpml_step(pm.psiH[2][1], qz, pz1, bz, cz, dzEx);
       }
-      if (GEN && pm.on) {
-        const int xsl = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
-        if (xsl >= 0) {  // x axis: slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
-          const float bb = pm.bH[0][i], cc = pm.cH[0][i];
-          const long long pq = psi_index(g, pm, 0, xsl, i, j, k);
-          cpml_apply(pm.psiH[0][0], pq, bb, cc, dxEz);
-          cpml_apply(pm.psiH[0][1], pq, bb, cc, dxEy);
-        }
-        if (ysl >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
-          const float bb = pm.bH[1][j], cc = pm.cH[1][j];
-          const long long pq = psi_index(g, pm, 1, ysl, i, j, k);
-          cpml_apply(pm.psiH[1][0], pq, bb, cc, dyEx);
-          cpml_apply(pm.psiH[1][1], pq, bb, cc, dyEz);
-        }
+      if (GX && xsl >= 0) {  // x axis: slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
+        cpml_step(pm.psiH[0][0], qx, px0, bx, cx, dxEz);
+        cpml_step(pm.psiH[0][1], qx, px1, bx, cx, dxEy);
       }
-      const long long q = pb + own;
+      if (GY && ysl >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
+        cpml_step(pm.psiH[1][0], qy, py0, by, cy, dyEx);
+        cpml_step(pm.psiH[1][1], qy, py1, by, cy, dyEz);
+      }
+      const long long q = plane_base<GX>(g, i) + own;
       st(Hx, q, hx0 - dt * (dyEz - dzEy), sc.sh[0]);
       st(Hy, q, hy0 - dt * (dzEx - dxEz), sc.sh[1]);
       st(Hz, q, hz0 - dt * (dxEy - dyEx), sc.sh[2]);
     }
-    // advance the queue to planes i-1 .. i+2
-    qy[0] = qy[1]; qy[1] = qy[2]; qy[2] = qy[3]; qy[3] = nqy;
-    qz[0] = qz[1]; qz[1] = qz[2]; qz[2] = qz[3]; qz[3] = nqz;
+  };
+  int i = t.i0;
+  for (; i + 1 < t.i1; i += 2) {
+    step(i, A);
+    step(i + 1, B);
   }
+  if (i < t.i1) step(i, A);
 }
 
 // ============================================================ ADE cells
@@ -454,11 +551,14 @@ __global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Fields<STO
 // MODE 2 (H pre-correction before a fused step, on H^{n-1/2}): the fused
 // step's own psi update then carries c*D, so psi gets c*D/b and H gets
 // -dt*s*D only.
-template <int STORE, int MODE>
+// One thread per cell of the union of the face bands: the cell belongs to
+// the first band that contains it, and that thread applies the terms of
+// every axis whose band holds the cell, so cells where bands of different
+// axes overlap (box edges and corners) are written by one thread only.
+template <int STORE, int MODE, bool TAB>
 __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt, Ade ade,
                                               Tfsf tf, TfsfBands bands, double tinc) {
   constexpr int HALF = MODE == 1 ? 1 : 0;
-  long long acc = 0;
   const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   int r = 0;
   long long loc = gid;
@@ -467,79 +567,125 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
     const long long cnt = (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
     if (loc < cnt) break;
     loc -= cnt;
-    acc += cnt;
   }
   if (r >= bands.n) return;
   const Region& R = bands.r[r];
-  const int a = bands.axis[r];
   const int nk = R.k1 - R.k0, nj = R.j1 - R.j0;
   const int k = R.k0 + (int)(loc % nk);
   const int j = R.j0 + (int)((loc / nk) % nj);
   const int i = R.i0 + (int)(loc / ((long long)nk * nj));
+  auto holds = [&](const Region& Q) {
+    return i >= Q.i0 && i < Q.i1 && j >= Q.j0 && j < Q.j1 && k >= Q.k0 && k < Q.k1;
+  };
+  for (int r2 = 0; r2 < r; r2++)
+    if (holds(bands.r[r2])) return;  // owned by an earlier band
   const long long q = i * g.sx + j * g.sy + k;
   const int vpos[3] = {i, j, k};
-  // components with a derivative along a: c1 = (a+2)%3 (first curl term),
-  // c2 = (a+1)%3 (second curl term); psi slots 1 and 0 of axis a.
   static constexpr int off[6][3] = {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  for (int t = 0; t < 2; t++) {
-    const int c = t == 0 ? (a + 2) % 3 : (a + 1) % 3;
-    const float sgn = t == 0 ? 1.0f : -1.0f;
-    const int slot = t == 0 ? 1 : 0;
-    const int comp = HALF == 0 ? 3 + c : c;
-    const int hx = 2 * i + off[comp][0], hy = 2 * j + off[comp][1], hz = 2 * k + off[comp][2];
-    const float D = g.rd[a] * tfsf_dcorr(tf, hx, hy, hz, a, tinc);
-    if (D == 0.0f) continue;
-    float cp = 0.0f;
-    if (pm.on) {
-      const int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
-      const int si = slab_index(vpos[a], n, pm.plo[a], pm.phi[a]);
-      if (si >= 0) {
-        cp = HALF == 0 ? pm.cH[a][vpos[a]] : pm.cE[a][vpos[a]];
-        float* psi = HALF == 0 ? pm.psiH[a][slot] : pm.psiE[a][slot];
-        const long long pq = psi_index(g, pm, a, si, i, j, k);
-        if (MODE == 2) {
-          psi[pq] += cp * D / pm.bH[a][vpos[a]];
-          cp = 0.0f;
-        } else {
-          psi[pq] += cp * D;
+  float dc[3] = {0.f, 0.f, 0.f};  // curl correction per component, all axes
+  for (int rb = r; rb < bands.n; rb++) {
+    if (rb != r && !holds(bands.r[rb])) continue;
+    const int a = bands.axis[rb];
+    // components with a derivative along a: c1 = (a+2)%3 (first curl term),
+    // c2 = (a+1)%3 (second curl term); psi slots 1 and 0 of axis a.
+    for (int t = 0; t < 2; t++) {
+      const int c = t == 0 ? (a + 2) % 3 : (a + 1) % 3;
+      const float sgn = t == 0 ? 1.0f : -1.0f;
+      const int slot = t == 0 ? 1 : 0;
+      const int comp = HALF == 0 ? 3 + c : c;
+      const int hx = 2 * i + off[comp][0], hy = 2 * j + off[comp][1], hz = 2 * k + off[comp][2];
+      const float D = g.rd[a] * tfsf_dcorr<TAB>(tf, hx, hy, hz, a, tinc);
+      if (D == 0.0f) continue;
+      float cp = 0.0f;
+      if (pm.on) {
+        const int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+        const int si = slab_index(vpos[a], n, pm.plo[a], pm.phi[a]);
+        if (si >= 0) {
+          cp = HALF == 0 ? pm.cH[a][vpos[a]] : pm.cE[a][vpos[a]];
+          float* psi = HALF == 0 ? pm.psiH[a][slot] : pm.psiE[a][slot];
+          const long long pq = psi_index(g, pm, a, si, i, j, k);
+          if (MODE == 2) {
+            psi[pq] += cp * D / pm.bH[a][vpos[a]];
+            cp = 0.0f;
+          } else {
+            psi[pq] += cp * D;
+          }
         }
       }
+      dc[c] += sgn * (1.0f + cp) * D;
     }
-    const float dcurl = sgn * (1.0f + cp) * D;
-    if (HALF == 0) {
-      st(f.H[c], q, ld(f.H[c], q, sc.ih[c]) - g.dt * dcurl, sc.sh[c]);
-      continue;
+  }
+  if (HALF == 0) {
+    for (int c = 0; c < 3; c++)
+      if (dc[c] != 0.0f) st(f.H[c], q, ld(f.H[c], q, sc.ih[c]) - g.dt * dc[c], sc.sh[c]);
+    return;
+  }
+  const ClassEntry ce = mt.table[mt.cls ? mt.cls[q] : 0];
+  bool over = false;
+  float dv = 0.f;
+  if (mt.d_on) {
+    const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
+    if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) {
+      dv = mt.d_inv_eps[((long long)di * mt.d_nj + dj) * mt.d_nk + dk];
+      over = true;
     }
-    const ClassEntry ce = mt.table[mt.cls ? mt.cls[q] : 0];
-    float ie = ce.inv_eps[c];
-    bool over = false;
-    if (mt.d_on) {
-      const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
-      if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) {
-        ie = mt.d_inv_eps[((long long)di * mt.d_nj + dj) * mt.d_nk + dk];
-        over = true;
-      }
-    }
+  }
+  for (int c = 0; c < 3; c++) {
+    if (dc[c] == 0.0f) continue;
     if (!over && (ce.special & (1 << c)) && ade.on) {
       const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
       const AdeMat& am = ade.mats[ce.mat[c]];
-      const float dE = am.kE[2] * g.dt * dcurl;
+      const float dE = am.kE[2] * g.dt * dc[c];
       st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + dE, sc.se[c]);
       for (int p = 0; p < am.np; p++) ade.Jp[c][p][b] += am.kJ[p][2] * dE;
     } else {
-      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + g.dt * ie * dcurl, sc.se[c]);
+      const float ie = over ? dv : ce.inv_eps[c];
+      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + g.dt * ie * dc[c], sc.se[c]);
     }
   }
 }
 
+// Plane-wave waveform along the propagation axis, one value per
+// half-coordinate: with k parallel to axis a the incident phase depends on
+// the half-coordinate along a only (signal.hpp:17-26).  One launch fills the
+// tables of nsteps steps, both half-steps: out[(s*2 + half)*tab_n + h] at
+// t = (n0+s) dt (H half, incident E) and (n0+s+1/2) dt (E half, incident H).
+__global__ __launch_bounds__(256) void k_tfsf_table(Tfsf tf, double* out, long long n0, int nsteps, double dt) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)nsteps * 2 * tf.tab_n) return;
+  const int h = (int)(idx % tf.tab_n);
+  const int half = (int)((idx / tf.tab_n) % 2);
+  const long long n = n0 + idx / (2LL * tf.tab_n);
+  const double t = half ? ((double)n + 0.5) * dt : (double)n * dt;
+  const int a = tf.tab_axis;
+  const double ka = a == 0 ? tf.kx : (a == 1 ? tf.ky : tf.kz);
+  const double xa = (h + tf.tab_h0) * tf.half_d[a];
+  double f0, f1, f2;
+  wave3(tf.waveform, tf.w * t - ka * (xa - tf.pos[a]), tf.w, tf.kEnv, tf.t0, tf.tw, f0, f1, f2);
+  out[idx] = tf.amp * f0;
+}
+
 // ================================================================ E sweep
-template <int STORE, bool GEN>
+// Operands of one plane, loaded two planes ahead: the Hx tile values of
+// plane p, the E read-modify-write operands and class byte of plane p, and
+// the Hy / Hz tile values of plane p+2 (the x stencil of E reaches two
+// planes ahead).
+template <typename T> struct PreE {
+  T hx, hxy, hxz, hy, hyz, hz, hzy, ex, ey, ez;
+  int cls;
+};
+
+template <int STORE, int GEN>
 __global__ __launch_bounds__(BZ* MBY) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
                                                       Ade ade, MarchSet ms) {
+  using T = typename Raw<STORE>::T;
+  constexpr bool GX = GEN & 1, GY = GEN & 2;
   __shared__ ClassEntry tbl[kMaxClasses];
-  __shared__ float sHx[2][MBY + 3][MLZ];  // [row j-j0+1][col k-k0+1]
-  __shared__ float sHy[2][MBY][MLZ];      // [row j-j0][col k-k0+1]
-  __shared__ float sHz[2][MBY + 3][BZ];   // [row j-j0+1][col k-k0]
+  __shared__ float sHx[2][MBY + 3][MLZ];  // [row j-j0+1][col k-k0+1], plane i
+  __shared__ float rHy[5][MBY][MLZ];      // [row j-j0][col k-k0+1], ring of planes
+  __shared__ float rHz[5][MBY + 3][BZ];   // [row j-j0+1][col k-k0], ring of planes
+  __shared__ float sEo[3][MBY][BZ];       // this thread's E operands of plane i
+  __shared__ int sCl[MBY][BZ];            // and its class byte
   const int tx = threadIdx.x, ty = threadIdx.y;
   for (int q = tx + ty * BZ; q < mt.nclass; q += BZ * MBY) tbl[q] = mt.table[q];
   MarchTile t;
@@ -548,127 +694,155 @@ __global__ __launch_bounds__(BZ* MBY) void k_update_e(Grid g, Fields<STORE> f, S
   const int j = t.j0 + ty, k = t.k0 + tx;
   const bool inreg = j < R.j1 && k < R.k1;
   const long long own = plane_off(g, j, k);
-  // halo rows j0-1, j0+MBY, j0+MBY+1 (ty<3); columns k0-1, k0+64, k0+65 (tx<3)
+  // halo rows j0-1, j0+MBY, j0+MBY+1 (waves ty<3); columns k0-1, k0+64, k0+65 (lanes tx<3)
+  const bool yw = ty < 3, zl = tx < 3;
   const int yrow = ty == 0 ? 0 : MBY + ty;
-  const long long yh = ty < 3 ? plane_off(g, t.j0 + (ty == 0 ? -1 : MBY + ty - 1), k) : -1;
+  const long long yh = yw ? plane_off(g, t.j0 + (ty == 0 ? -1 : MBY + ty - 1), k) : -1;
   const int zcol = tx == 0 ? 0 : BZ + tx;
-  const long long zh = tx < 3 ? plane_off(g, j, t.k0 + (tx == 0 ? -1 : BZ + tx - 1)) : -1;
-  const auto* Hx = f.H[0];
-  const auto* Hy = f.H[1];
-  const auto* Hz = f.H[2];
-  auto* Ex = f.E[0];
-  auto* Ey = f.E[1];
-  auto* Ez = f.E[2];
+  const long long zh = zl ? plane_off(g, j, t.k0 + (tx == 0 ? -1 : BZ + tx - 1)) : -1;
+  const T* Hx = f.H[0];
+  const T* Hy = f.H[1];
+  const T* Hz = f.H[2];
+  T* Ex = f.E[0];
+  T* Ey = f.E[1];
+  T* Ez = f.E[2];
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
-  // x queue: Hy, Hz at planes i-1 .. i+2
-  float qy[4], qz[4];
-  for (int s = 0; s < 4; s++) {
-    const long long pb = plane_base<GEN>(g, t.i0 - 1 + s);
-    qy[s] = ldp(Hy, pb, own, ihy);
-    qz[s] = ldp(Hz, pb, own, ihz);
-  }
+  const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
+  const T z0 = rzero<T>();
+
+  const long long eq = inreg ? own : -1;  // E operands: owned cells only
+  const uint8_t* clsp = mt.cls ? mt.cls : static_cast<const uint8_t*>(g.zero);
+  const long long cq = mt.cls ? eq : -1;
+  auto fetch = [&](PreE<T>& S, int p) {
+    const long long pb = plane_base<GX>(g, p);
+    const long long pn = plane_base<GX>(g, p + 2);
+    S.hx = *at(Hx, pb, own, g.zero);
+    S.hy = *at(Hy, pn, own, g.zero);
+    S.hz = *at(Hz, pn, own, g.zero);
+    S.hxy = *at(Hx, pb, yh, g.zero);
+    S.hzy = *at(Hz, pn, yh, g.zero);
+    S.hxz = *at(Hx, pb, zh, g.zero);
+    S.hyz = *at(Hy, pn, zh, g.zero);
+    S.ex = *at((const T*)Ex, pb, eq, g.zero);
+    S.ey = *at((const T*)Ey, pb, eq, g.zero);
+    S.ez = *at((const T*)Ez, pb, eq, g.zero);
+    S.cls = *at(clsp, pb, cq, g.zero);
+  };
+  auto ring_direct = [&](int p) {  // Hy / Hz tile of plane p into its slot
+    const long long pb = plane_base<GX>(g, p);
+    const int s = ring5(p);
+    rHy[s][ty][tx + 1] = cvt(ldr(Hy, pb, own), ihy);
+    rHz[s][ty + 1][tx] = cvt(ldr(Hz, pb, own), ihz);
+    if (yw) rHz[s][yrow][tx] = cvt(ldr(Hz, pb, yh), ihz);
+    if (zl) rHy[s][ty][zcol] = cvt(ldr(Hy, pb, zh), ihy);
+  };
+
   const int zs = pm.on ? slab_index(k, g.nz, pm.plo[2], pm.phi[2]) : -1;
-  const int ysl = (GEN && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
+  const int ysl = (GY && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
   const bool design_col = mt.d_on && j >= mt.d_j0 && j < mt.d_j0 + mt.d_nj && k >= mt.d_k0 && k < mt.d_k0 + mt.d_nk;
   const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
-  // plane-i operands, loaded one plane ahead: own Hx, halo values, the E
-  // read-modify-write operands and the material class byte
-  float n_hx, n_y0 = 0.f, n_y1 = 0.f, n_z0 = 0.f, n_z1 = 0.f, n_ex = 0.f, n_ey = 0.f, n_ez = 0.f;
-  int n_cls = 0;
-  {
-    const long long pb = plane_base<GEN>(g, t.i0);
-    n_hx = ldp(Hx, pb, own, ihx);
-    if (ty < 3) { n_y0 = ldp(Hx, pb, yh, ihx); n_y1 = ldp(Hz, pb, yh, ihz); }
-    if (tx < 3) { n_z0 = ldp(Hx, pb, zh, ihx); n_z1 = ldp(Hy, pb, zh, ihy); }
-    if (inreg) {
-      n_ex = ld(Ex, pb + own, sc.ie[0]); n_ey = ld(Ey, pb + own, sc.ie[1]); n_ez = ld(Ez, pb + own, sc.ie[2]);
-      n_cls = mt.cls ? mt.cls[pb + own] : 0;
-    }
-  }
-  for (int i = t.i0; i < t.i1; i++) {
+  const float bz = zs >= 0 ? pm.bE[2][k] : 0.f, cz = zs >= 0 ? pm.cE[2][k] : 0.f;
+  const float by = ysl >= 0 ? pm.bE[1][j] : 0.f, cy = ysl >= 0 ? pm.cE[1][j] : 0.f;
+
+  PreE<T> A{z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, 0}, B = A;
+  ring_direct(t.i0 - 1);
+  ring_direct(t.i0);
+  ring_direct(t.i0 + 1);
+  fetch(A, t.i0);  // unconditional prefetches, as in the H sweep
+  fetch(B, min(t.i0 + 1, t.i1 - 1));
+
+  auto step = [&](int i, PreE<T>& S) {
     const int b = i & 1;
-    const long long pb = plane_base<GEN>(g, i);
-    const float hxo = n_hx, ex0 = n_ex, ey0 = n_ey, ez0 = n_ez;
-    const int cls = n_cls;
-    sHx[b][ty + 1][tx + 1] = hxo;
-    sHy[b][ty][tx + 1] = qy[1];
-    sHz[b][ty + 1][tx] = qz[1];
-    if (ty < 3) {
-      sHx[b][yrow][tx + 1] = n_y0;
-      sHz[b][yrow][tx] = n_y1;
+    const int s0 = ring5(i - 1), s1 = ring5(i), s2 = ring5(i + 1), s3 = ring5(i + 2);
+    // publish plane i (Hx) and plane i+2 (Hy, Hz); E operands to the own slot
+    sHx[b][ty + 1][tx + 1] = cvt(S.hx, ihx);
+    rHy[s3][ty][tx + 1] = cvt(S.hy, ihy);
+    rHz[s3][ty + 1][tx] = cvt(S.hz, ihz);
+    if (yw) {
+      sHx[b][yrow][tx + 1] = cvt(S.hxy, ihx);
+      rHz[s3][yrow][tx] = cvt(S.hzy, ihz);
     }
-    if (tx < 3) {
-      sHx[b][ty + 1][zcol] = n_z0;
-      sHy[b][ty][zcol] = n_z1;
+    if (zl) {
+      sHx[b][ty + 1][zcol] = cvt(S.hxz, ihx);
+      rHy[s3][ty][zcol] = cvt(S.hyz, ihy);
     }
-    // prefetch plane i+1 operands and the queue's plane i+3
-    float nqy = 0.f, nqz = 0.f;
-    if (i + 1 < t.i1) {
-      const long long p1 = plane_base<GEN>(g, i + 1);
-      const long long p3 = plane_base<GEN>(g, i + 3);
-      n_hx = ldp(Hx, p1, own, ihx);
-      if (ty < 3) { n_y0 = ldp(Hx, p1, yh, ihx); n_y1 = ldp(Hz, p1, yh, ihz); }
-      if (tx < 3) { n_z0 = ldp(Hx, p1, zh, ihx); n_z1 = ldp(Hy, p1, zh, ihy); }
-      if (inreg) {
-        n_ex = ld(Ex, p1 + own, sc.ie[0]); n_ey = ld(Ey, p1 + own, sc.ie[1]); n_ez = ld(Ez, p1 + own, sc.ie[2]);
-        n_cls = mt.cls ? mt.cls[p1 + own] : 0;
-      }
-      nqy = ldp(Hy, p3, own, ihy);
-      nqz = ldp(Hz, p3, own, ihz);
+    sEo[0][ty][tx] = cvt(S.ex, iex);
+    sEo[1][ty][tx] = cvt(S.ey, iey);
+    sEo[2][ty][tx] = cvt(S.ez, iez);
+    sCl[ty][tx] = S.cls;
+    // psi and design-box operands of plane i before the prefetch
+    float pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f, py0 = 0.f, py1 = 0.f, dv = 0.f;
+    long long qz = 0, qx = 0, qy = 0;
+    const int xsl = (GX && pm.on) ? slab_index(i, g.nx, pm.plo[0], pm.phi[0]) : -1;
+    if (inreg && zs >= 0) {
+      qz = psi_index(g, pm, 2, zs, i, j, k);
+      pz0 = pm.psiE[2][0][qz];
+      pz1 = pm.psiE[2][1][qz];
     }
+    float bx = 0.f, cx = 0.f;
+    if (GX && xsl >= 0) {
+      bx = pm.bE[0][i];
+      cx = pm.cE[0][i];
+    }
+    if (GX && inreg && xsl >= 0) {
+      qx = psi_index(g, pm, 0, xsl, i, j, k);
+      px0 = pm.psiE[0][0][qx];
+      px1 = pm.psiE[0][1][qx];
+    }
+    if (GY && inreg && ysl >= 0) {
+      qy = psi_index(g, pm, 1, ysl, i, j, k);
+      py0 = pm.psiE[1][0][qy];
+      py1 = pm.psiE[1][1][qy];
+    }
+    const bool over = inreg && design_col && i >= mt.d_i0 && i < mt.d_i0 + mt.d_ni;
+    if (over) dv = mt.d_inv_eps[((long long)(i - mt.d_i0) * mt.d_nj + (j - mt.d_j0)) * mt.d_nk + (k - mt.d_k0)];
+    fetch(S, min(i + 2, t.i1 - 1));
     lds_barrier();
     if (inreg) {
-      float dyHz = rd1 * d24(sHz[b][ty][tx], qz[1], sHz[b][ty + 2][tx], sHz[b][ty + 3][tx]);
+      const float ex0 = sEo[0][ty][tx], ey0 = sEo[1][ty][tx], ez0 = sEo[2][ty][tx];
+      const int cls = sCl[ty][tx];
+      const float hxo = sHx[b][ty + 1][tx + 1];
+      float dyHz = rd1 * d24(rHz[s1][ty][tx], rHz[s1][ty + 1][tx], rHz[s1][ty + 2][tx], rHz[s1][ty + 3][tx]);
       float dyHx = rd1 * d24(sHx[b][ty][tx + 1], hxo, sHx[b][ty + 2][tx + 1], sHx[b][ty + 3][tx + 1]);
-      float dzHy = rd2 * d24(sHy[b][ty][tx], qy[1], sHy[b][ty][tx + 2], sHy[b][ty][tx + 3]);
+      float dzHy = rd2 * d24(rHy[s1][ty][tx], rHy[s1][ty][tx + 1], rHy[s1][ty][tx + 2], rHy[s1][ty][tx + 3]);
       float dzHx = rd2 * d24(sHx[b][ty + 1][tx], hxo, sHx[b][ty + 1][tx + 2], sHx[b][ty + 1][tx + 3]);
-      float dxHz = rd0 * d24(qz[0], qz[1], qz[2], qz[3]);
-      float dxHy = rd0 * d24(qy[0], qy[1], qy[2], qy[3]);
+      float dxHz = rd0 * d24(rHz[s0][ty + 1][tx], rHz[s1][ty + 1][tx], rHz[s2][ty + 1][tx], rHz[s3][ty + 1][tx]);
+      float dxHy = rd0 * d24(rHy[s0][ty][tx + 1], rHy[s1][ty][tx + 1], rHy[s2][ty][tx + 1], rHy[s3][ty][tx + 1]);
       if (zs >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
-        const float bb = pm.bE[2][k], cc = pm.cE[2][k];
-        const long long pq = psi_index(g, pm, 2, zs, i, j, k);
-        cpml_apply(pm.psiE[2][0], pq, bb, cc, dzHy);
-        cpml_apply(pm.psiE[2][1], pq, bb, cc, dzHx);
+        cpml_step(pm.psiE[2][0], qz, pz0, bz, cz, dzHy);
+        cpml_step(pm.psiE[2][1], qz, pz1, bz, cz, dzHx);
       }
-      if (GEN && pm.on) {
-        const int xsl = slab_index(i, g.nx, pm.plo[0], pm.phi[0]);
-        if (xsl >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
-          const float bb = pm.bE[0][i], cc = pm.cE[0][i];
-          const long long pq = psi_index(g, pm, 0, xsl, i, j, k);
-          cpml_apply(pm.psiE[0][0], pq, bb, cc, dxHz);
-          cpml_apply(pm.psiE[0][1], pq, bb, cc, dxHy);
-        }
-        if (ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
-          const float bb = pm.bE[1][j], cc = pm.cE[1][j];
-          const long long pq = psi_index(g, pm, 1, ysl, i, j, k);
-          cpml_apply(pm.psiE[1][0], pq, bb, cc, dyHx);
-          cpml_apply(pm.psiE[1][1], pq, bb, cc, dyHz);
-        }
+      if (GX && xsl >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
+        cpml_step(pm.psiE[0][0], qx, px0, bx, cx, dxHz);
+        cpml_step(pm.psiE[0][1], qx, px1, bx, cx, dxHy);
       }
-      const float cx = dyHz - dzHy, cy = dzHx - dxHz, cz = dxHy - dyHx;
-      const long long q = pb + own;
+      if (GY && ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
+        cpml_step(pm.psiE[1][0], qy, py0, by, cy, dyHx);
+        cpml_step(pm.psiE[1][1], qy, py1, by, cy, dyHz);
+      }
+      const float crx = dyHz - dzHy, cry = dzHx - dxHz, crz = dxHy - dyHx;
+      const long long q = plane_base<GX>(g, i) + own;
       const ClassEntry& ce = tbl[cls];
       float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
-      bool over = false;
-      if (design_col && i >= mt.d_i0 && i < mt.d_i0 + mt.d_ni) {
-        const float v = mt.d_inv_eps[((long long)(i - mt.d_i0) * mt.d_nj + (j - mt.d_j0)) * mt.d_nk + (k - mt.d_k0)];
-        ie0 = ie1 = ie2 = v;
-        over = true;
-      }
+      if (over) ie0 = ie1 = ie2 = dv;
       if (ce.special && !over) {  // finished by k_ade
         const long long bq = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
-        ade.curl[0][bq] = cx;
-        ade.curl[1][bq] = cy;
-        ade.curl[2][bq] = cz;
+        ade.curl[0][bq] = crx;
+        ade.curl[1][bq] = cry;
+        ade.curl[2][bq] = crz;
       } else {
-        st(Ex, q, ex0 + dt * ie0 * cx, sc.se[0]);
-        st(Ey, q, ey0 + dt * ie1 * cy, sc.se[1]);
-        st(Ez, q, ez0 + dt * ie2 * cz, sc.se[2]);
+        st(Ex, q, ex0 + dt * ie0 * crx, sc.se[0]);
+        st(Ey, q, ey0 + dt * ie1 * cry, sc.se[1]);
+        st(Ez, q, ez0 + dt * ie2 * crz, sc.se[2]);
       }
     }
-    qy[0] = qy[1]; qy[1] = qy[2]; qy[2] = qy[3]; qy[3] = nqy;
-    qz[0] = qz[1]; qz[1] = qz[2]; qz[2] = qz[3]; qz[3] = nqz;
+  };
+  int i = t.i0;
+  for (; i + 1 < t.i1; i += 2) {
+    step(i, A);
+    step(i + 1, B);
   }
+  if (i < t.i1) step(i, A);
 }
 
 // ======================================================== small kernels
@@ -766,9 +940,27 @@ static void check_launch(const char* what) {
   if (e != hipSuccess) fprintf(stderr, "dtm: launch of %s failed: %s\n", what, hipGetErrorString(e));
 }
 
-static int march_seg(const Region* regs, int nreg) {
-  // Segments long enough to amortise the 3-plane queue warm-up, short enough
-  // to give every CU several blocks.
+// Blocks of one launch that fit on the device at once.
+template <typename K>
+static int resident_blocks(K kernel) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return 512;
+    cus = p.multiProcessorCount;
+  }
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BZ * MBY, 0) != hipSuccess || per < 1) per = 1;
+  return per * cus;
+}
+
+// Segment length along x: a block marches seg planes after a warm-up of
+// about kWarm planes (three ring planes loaded synchronously, then the
+// two-plane prefetch filling), and the launch runs in ceil(blocks / slots)
+// rounds.  Minimise rounds * (seg + kWarm).
+static int march_seg(const Region* regs, int nreg, int slots) {
+  constexpr int kWarm = 6;
   long long cols = 0;
   int maxlen = 1;
   for (int t = 0; t < nreg; t++) {
@@ -777,9 +969,28 @@ static int march_seg(const Region* regs, int nreg) {
     cols += (long long)((r.k1 - r.k0 + BZ - 1) / BZ) * ((r.j1 - r.j0 + MBY - 1) / MBY);
     maxlen = std::max(maxlen, r.i1 - r.i0);
   }
-  int seg = 32;
-  while (seg > 8 && cols * ((maxlen + seg - 1) / seg) < 4096) seg /= 2;
-  return seg;
+  int best = maxlen;
+  long long best_cost = -1;
+  for (int nseg = 1; nseg <= maxlen; nseg++) {
+    const int seg = (maxlen + nseg - 1) / nseg;
+    if (nseg > 1 && seg == (maxlen + nseg - 2) / (nseg - 1)) continue;  // same seg as nseg-1
+    long long blocks = 0;
+    for (int t = 0; t < nreg; t++) {
+      const Region& r = regs[t];
+      if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
+      blocks += (long long)((r.k1 - r.k0 + BZ - 1) / BZ) * ((r.j1 - r.j0 + MBY - 1) / MBY) *
+                ((r.i1 - r.i0 + seg - 1) / seg);
+    }
+    const long long rounds = (blocks + slots - 1) / slots;
+    const long long cost = rounds * (seg + kWarm);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = seg;
+    }
+    if (seg <= 4) break;
+  }
+  (void)cols;
+  return best;
 }
 
 template <int STORE>
@@ -787,15 +998,17 @@ static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpm
                   const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, MBY, 1);
+  static const int slots_m = resident_blocks(k_update_h<STORE, 1>);
+  static const int slots_e = resident_blocks(k_update_h<STORE, 3>);
   const int nm = main.i1 > main.i0 ? 1 : 0;
-  MarchSet rm = make_march(&main, nm, march_seg(&main, nm));
+  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m));
   if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, rm);
+    hipLaunchKernelGGL((k_update_h<STORE, 1>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, rm);
     check_launch("k_update_h<main>");
   }
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg));
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e));
   if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
+    hipLaunchKernelGGL((k_update_h<STORE, 3>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
     check_launch("k_update_h<edge>");
   }
 }
@@ -805,15 +1018,17 @@ static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpm
                   const Ade& ade, const Region* regs, int nreg, const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, MBY, 1);
+  static const int slots_m = resident_blocks(k_update_e<STORE, 1>);
+  static const int slots_e = resident_blocks(k_update_e<STORE, 3>);
   const int nm = main.i1 > main.i0 ? 1 : 0;
-  MarchSet rm = make_march(&main, nm, march_seg(&main, nm));
+  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m));
   if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, false>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rm);
+    hipLaunchKernelGGL((k_update_e<STORE, 1>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rm);
     check_launch("k_update_e<main>");
   }
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg));
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e));
   if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, true>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
+    hipLaunchKernelGGL((k_update_e<STORE, 3>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
     check_launch("k_update_e<edge>");
   }
 }
@@ -838,22 +1053,24 @@ template <int STORE, int MODE>
 static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                     const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double tinc, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
-  for (int a = 0; a < 3; a++) {
-    TfsfBands sub{};
-    long long n = 0;
-    for (int r = 0; r < bands.n; r++) {
-      if (bands.axis[r] != a) continue;
-      const Region& R = bands.r[r];
-      sub.r[sub.n] = R;
-      sub.axis[sub.n] = a;
-      sub.n++;
-      n += (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
-    }
-    if (n <= 0) continue;
-    hipLaunchKernelGGL((k_tfsf<STORE, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, f, sc, pm, mt,
-                       ade, tf, sub, tinc);
-    check_launch("k_tfsf");
+  long long n = 0;
+  for (int r = 0; r < bands.n; r++) {
+    const Region& R = bands.r[r];
+    n += (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
   }
+  if (n <= 0) return;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (tf.tab_axis >= 0)
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE, true>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, bands, tinc);
+  else
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE, false>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, bands, tinc);
+  check_launch("k_tfsf");
+}
+
+void launch_tfsf_tables(const Tfsf& tf, double* out, long long n0, int nsteps, double dt, hipStream_t st) {
+  const long long n = (long long)nsteps * 2 * tf.tab_n;
+  hipLaunchKernelGGL(k_tfsf_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tf, out, n0, nsteps, dt);
+  check_launch("k_tfsf_table");
 }
 
 void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,

@@ -26,6 +26,7 @@
 using namespace dtm;
 
 static thread_local std::string g_err;
+static constexpr int kTableSteps = 32;  // plane-wave waveform tables per batch
 
 static int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -55,9 +56,14 @@ struct dtm_solver {
   Cpml pmn{};        // second psi buffers (torre step)
   bool torre = false; // fused single-pass step; false: two sweeps per step
   std::vector<void*> allocs;  // everything to free
-  Region main{};
+  std::vector<std::pair<void*, size_t>> state;  // field / psi buffers (zeroed by a reset)
+  bool adjoint = false;  // stepping adjoint fields: no sources, no recording
+  Region main{};       // fused step: interior box
   Region edge[4]{};
   int nedge = 0;
+  Region main2{};      // two-pass sweeps: all x, y outside the y CPML
+  Region edge2[2]{};
+  int nedge2 = 0;
   // materials
   std::vector<uint8_t> cls_host;
   std::vector<ClassEntry> classes;
@@ -73,6 +79,8 @@ struct dtm_solver {
   // TF/SF
   Tfsf tf{};
   TfsfBands bands{};
+  double* tab_base = nullptr;  // waveform tables of kTableSteps steps x 2 halves
+  long long tab_n0 = -1;       // first step held in tab_base
   // probes
   std::vector<ProbeDesc> probes;
   ProbeDesc* probes_dev = nullptr;
@@ -106,6 +114,19 @@ static int dalloc(dtm_solver* s, T** p, size_t bytes) {
   if (e != hipSuccess) return fail(DTM_ERR_DEVICE, hipGetErrorString(e));
   s->allocs.push_back(q);
   *p = static_cast<T*>(q);
+  return 0;
+}
+
+// dalloc for buffers that hold time-stepped state.
+template <typename T>
+static int sdalloc(dtm_solver* s, T** p, size_t bytes) {
+  if (int rc = dalloc(s, p, bytes)) return rc;
+  s->state.emplace_back(static_cast<void*>(*p), bytes == 0 ? 16 : bytes);
+  return 0;
+}
+
+static int zero_state(dtm_solver* s) {
+  for (auto& b : s->state) HIPCHK(hipMemsetAsync(b.first, 0, b.second, s->stream));
   return 0;
 }
 
@@ -346,6 +367,31 @@ static int setup_tfsf(dtm_solver* s) {
     t.pos[a] = c.tfsf.pos[a];
     t.dip[a] = 0.5 * (c.tfsf.box[2 * a] + c.tfsf.box[2 * a + 1]);
   }
+  // A plane wave whose k lies along one axis (the other components below
+  // 1e-12 |k|, e.g. cos(pi/2) rounding) has a phase that depends on that
+  // axis only: tabulate the waveform once per half-step.
+  t.tab_axis = -1;
+  if (t.kind == DTM_TFSF_PLANE) {
+    const double ka[3] = {fabs(t.kx), fabs(t.ky), fabs(t.kz)};
+    for (int a = 0; a < 3; a++) {
+      const int b = (a + 1) % 3, e = (a + 2) % 3;
+      if (ka[a] > 0 && ka[b] <= 1e-12 * ka[a] && ka[e] <= 1e-12 * ka[a]) t.tab_axis = a;
+    }
+    if (t.tab_axis >= 0) {
+      t.tab_h0 = -8;
+      t.tab_n = 2 * n[t.tab_axis] + 16;
+      if (int rc = dalloc(s, &s->tab_base, sizeof(double) * 2 * kTableSteps * t.tab_n)) return rc;
+      // incident component = P[c] * amp*f0(phase) (signal.hpp:28-49 with
+      // E_phi = Imp cos(psi) s, E_theta = Imp sin(psi) s, H_theta = -cos(psi) s,
+      // H_phi = sin(psi) s)
+      t.P[0] = t.Imp * (-t.cps * t.sph + t.sps * t.ct * t.cph);
+      t.P[1] = t.Imp * (t.cps * t.cph + t.sps * t.ct * t.sph);
+      t.P[2] = -t.Imp * t.sps * t.st;
+      t.P[3] = -t.sps * t.sph - t.cps * t.ct * t.cph;
+      t.P[4] = t.sps * t.cph - t.cps * t.ct * t.sph;
+      t.P[5] = t.cps * t.st;
+    }
+  }
   // face bands, clamped to the grid; lo/hi merged when they meet
   TfsfBands& B = s->bands;
   B.n = 0;
@@ -378,6 +424,20 @@ static int setup_tfsf(dtm_solver* s) {
     }
   }
   return 0;
+}
+
+// Tfsf of one TF/SF launch: for a tabulated plane wave, points the table
+// at step n, half (0: H half-step, 1: E half-step), filling the next batch
+// of tables first when step n is not in the current one.
+static Tfsf tfsf_for(dtm_solver* s, long long n, int half) {
+  Tfsf t = s->tf;
+  if (t.tab_axis < 0) return t;
+  if (s->tab_n0 < 0 || n < s->tab_n0 || n >= s->tab_n0 + kTableSteps) {
+    s->tab_n0 = n;
+    launch_tfsf_tables(s->tf, s->tab_base, n, kTableSteps, s->cfg.dt, s->stream);
+  }
+  t.table = s->tab_base + ((n - s->tab_n0) * 2 + half) * (long long)t.tab_n;
+  return t;
 }
 
 static bool tfsf_gate(const dtm_solver* s, long long n) {
@@ -425,8 +485,8 @@ static int setup_cpml(dtm_solver* s) {
     size_t cnt = 0;
     if (P > 0) cnt = (size_t)P * (a == 0 ? (size_t)c.ny * c.nz : a == 1 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny);
     for (int sl = 0; sl < 2; sl++) {
-      if (int rc = dalloc(s, &p.psiE[a][sl], sizeof(float) * cnt)) return rc;
-      if (int rc = dalloc(s, &p.psiH[a][sl], sizeof(float) * cnt)) return rc;
+      if (int rc = sdalloc(s, &p.psiE[a][sl], sizeof(float) * cnt)) return rc;
+      if (int rc = sdalloc(s, &p.psiH[a][sl], sizeof(float) * cnt)) return rc;
     }
   }
   s->pmn = p;
@@ -436,8 +496,8 @@ static int setup_cpml(dtm_solver* s) {
       size_t cnt = 0;
       if (P > 0) cnt = (size_t)P * (a == 0 ? (size_t)c.ny * c.nz : a == 1 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny);
       for (int sl = 0; sl < 2; sl++) {
-        if (int rc = dalloc(s, &s->pmn.psiE[a][sl], sizeof(float) * cnt)) return rc;
-        if (int rc = dalloc(s, &s->pmn.psiH[a][sl], sizeof(float) * cnt)) return rc;
+        if (int rc = sdalloc(s, &s->pmn.psiE[a][sl], sizeof(float) * cnt)) return rc;
+        if (int rc = sdalloc(s, &s->pmn.psiH[a][sl], sizeof(float) * cnt)) return rc;
       }
     }
   }
@@ -464,6 +524,13 @@ static void setup_regions(dtm_solver* s) {
   s->edge[s->nedge++] = Region{x1, c.nx, 0, c.ny, 0, c.nz};
   s->edge[s->nedge++] = Region{x0, x1, 0, y0, 0, c.nz};
   s->edge[s->nedge++] = Region{x0, x1, y1, c.ny, 0, c.nz};
+  // Two-pass sweeps: the main launch spans x (its x boundary and x CPML are
+  // per plane) and leaves the y CPML slabs to the edge launch.
+  const int yl = std::min(c.npml[2], c.ny), yh = std::max(c.ny - c.npml[3], yl);
+  s->main2 = Region{0, c.nx, yl, yh, 0, c.nz};
+  s->nedge2 = 0;
+  if (yl > 0) s->edge2[s->nedge2++] = Region{0, c.nx, 0, yl, 0, c.nz};
+  if (yh < c.ny) s->edge2[s->nedge2++] = Region{0, c.nx, yh, c.ny, 0, c.nz};
 }
 
 static void field_structs(dtm_solver* s) {
@@ -585,13 +652,13 @@ static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
     }
     for (int q = 0; q < 3; q++) {
       if (int rc = dalloc(s, &A.curl[q], sizeof(float) * nb)) return rc;
-      if (int rc = dalloc(s, &A.Em[q], sizeof(float) * nb)) return rc;
+      if (int rc = sdalloc(s, &A.Em[q], sizeof(float) * nb)) return rc;
       for (int p = 0; p < maxp; p++) {
-        if (int rc = dalloc(s, &A.Jp[q][p], sizeof(float) * nb)) return rc;
-        if (int rc = dalloc(s, &A.Jm[q][p], sizeof(float) * nb)) return rc;
+        if (int rc = sdalloc(s, &A.Jp[q][p], sizeof(float) * nb)) return rc;
+        if (int rc = sdalloc(s, &A.Jm[q][p], sizeof(float) * nb)) return rc;
       }
       if (any_gyro)
-        if (int rc = dalloc(s, &A.Jg[q], sizeof(float) * nb)) return rc;
+        if (int rc = sdalloc(s, &A.Jg[q], sizeof(float) * nb)) return rc;
     }
     if (int rc = dalloc(s, &s->ademats_dev, sizeof(AdeMat) * DTM_MAX_MATS)) return rc;
     if (int rc = dalloc(s, &s->gyromats_dev, sizeof(GyroMat) * DTM_MAX_MATS)) return rc;
@@ -675,14 +742,22 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
     s->sc.sh[a] = ldexpf(1.f, sh);
     s->sc.ih[a] = ldexpf(1.f, -sh);
   }
+  {
+    void* z = nullptr;
+    if (int rc = dalloc(s, &z, 256)) {
+      dtm_destroy(s);
+      return rc;
+    }
+    g.zero = z;
+  }
   const size_t n = (size_t)c.nx * c.ny * c.nz;
   const size_t esz = s->store ? 2 : 4;
   int rc = 0;
   for (int q = 0; q < 3 && !rc; q++) {
-    rc = dalloc(s, &s->E[q], esz * n);
-    if (!rc) rc = dalloc(s, &s->H[q], esz * n);
-    if (!rc && s->torre) rc = dalloc(s, &s->E2[q], esz * n);
-    if (!rc && s->torre) rc = dalloc(s, &s->H2[q], esz * n);
+    rc = sdalloc(s, &s->E[q], esz * n);
+    if (!rc) rc = sdalloc(s, &s->H[q], esz * n);
+    if (!rc && s->torre) rc = sdalloc(s, &s->E2[q], esz * n);
+    if (!rc && s->torre) rc = sdalloc(s, &s->H2[q], esz * n);
   }
   if (!rc) rc = setup_cpml(s);
   if (!rc) rc = setup_tfsf(s);
@@ -848,14 +923,14 @@ static void ev_end(dtm_solver* s) {
 static int one_step(dtm_solver* s) {
   const dtm_config& c = s->cfg;
   const long long n = s->step;
-  s->tf.active = tfsf_gate(s, n) ? 1 : 0;
+  s->tf.active = (!s->adjoint && tfsf_gate(s, n)) ? 1 : 0;
   const void* F = fields_ptr(s);
   const double th = (double)n * c.dt, te = ((double)n + 0.5) * c.dt;
   if (s->torre) {
     // TF/SF on H^{n-1/2} before the fused step (exact: the step is linear)
     if (s->tf.active) {
       ev_begin(s, 4);
-      launch_tfsf(s->store, 2, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, th, s->stream);
+      launch_tfsf(s->store, 2, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 0), s->bands, th, s->stream);
       ev_end(s);
     }
     ev_begin(s, 0);
@@ -870,32 +945,34 @@ static int one_step(dtm_solver* s) {
     launch_ade(s->store, s->g, F, fields_next_ptr(s), s->sc, s->mt, s->ade, s->stream);
     swap_buffers(s);
     F = fields_ptr(s);
-    if (s->tf.active) launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, te, s->stream);
+    if (s->tf.active)
+      launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
   } else {
     // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
     ev_begin(s, 0);
-    launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main, s->stream);
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main2, s->stream);
     ev_end(s);
     ev_begin(s, 1);
-    launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge, s->nedge, kEmpty, s->stream);
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge2, s->nedge2, kEmpty, s->stream);
     ev_end(s);
     if (s->tf.active) {
       ev_begin(s, 4);
-      launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, th, s->stream);
+      launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 0), s->bands, th, s->stream);
       ev_end(s);
     }
     // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
     ev_begin(s, 2);
-    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main, s->stream);
+    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main2, s->stream);
     ev_end(s);
     ev_begin(s, 3);
-    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge, s->nedge, kEmpty, s->stream);
+    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge2, s->nedge2, kEmpty, s->stream);
     ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, F, s->sc, s->mt, s->ade, s->stream);
-    if (s->tf.active) launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, s->tf, s->bands, te, s->stream);
+    if (s->tf.active)
+      launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
   }
-  if (c.nsoft > 0) {
+  if (c.nsoft > 0 && !s->adjoint) {
     Soft so{};
     so.n = c.nsoft;
     const double th = ((double)n + 0.5) * c.dt;
@@ -916,7 +993,7 @@ static int one_step(dtm_solver* s) {
     }
     launch_soft(s->store, s->g, F, s->sc, so, s->stream);
   }
-  const int np = (int)s->probes.size();
+  const int np = s->adjoint ? 0 : (int)s->probes.size();
   if (np > 0) {
     Probes pr{};
     pr.n = np;
@@ -985,6 +1062,118 @@ int dtm_dft_read(dtm_solver* s, double* out) {
 }
 
 static void* comp_ptr(dtm_solver* s, int comp) { return comp < 3 ? s->E[comp] : s->H[comp - 3]; }
+
+int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmon, const int* mon_ijk,
+                                double* J_out, double* grad_inv_eps) {
+  if (!s || nsteps < 1 || nmon < 1 || !mon_ijk || !J_out || !grad_inv_eps)
+    return fail(DTM_ERR_ARG, "bad adjoint arguments");
+  const dtm_config& c = s->cfg;
+  if (s->store != DTM_STORE_FP32) return fail(DTM_ERR_ARG, "the adjoint needs FP32 storage");
+  if (s->ade.on) return fail(DTM_ERR_ARG, "the adjoint does not cover dispersive or gyrotropic cells");
+  if (!s->mt.d_on) return fail(DTM_ERR_STATE, "no design box (dtm_set_design_eps)");
+  if (s->step != 0) return fail(DTM_ERR_STATE, "the adjoint starts at step 0");
+  if (int rc = start(s)) return rc;
+  const Materials& mt = s->mt;
+  const long long nd = (long long)mt.d_ni * mt.d_nj * mt.d_nk;
+  std::vector<long long> q(nmon);
+  std::vector<float> ie(nmon);
+  for (int m = 0; m < nmon; m++) {
+    const int i = mon_ijk[3 * m], j = mon_ijk[3 * m + 1], k = mon_ijk[3 * m + 2];
+    if (i < 0 || i >= c.nx || j < 0 || j >= c.ny || k < 0 || k >= c.nz) return fail(DTM_ERR_ARG, "monitor outside grid");
+    q[m] = ((long long)i * c.ny + j) * c.nz + k;
+    ie[m] = s->classes[s->cls_host[q[m]]].inv_eps[2];
+    const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
+    if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk)
+      ie[m] = s->design_host[((size_t)di * mt.d_nj + dj) * mt.d_nk + dk];
+  }
+  // temporaries: monitor cells, the Ez series, the design-box E history
+  // (nsteps+1 snapshots), the gradient accumulator
+  const size_t hist_bytes = sizeof(float) * 3 * (size_t)nd * ((size_t)nsteps + 1);
+  void *d_q = nullptr, *d_ie = nullptr, *d_ser = nullptr, *d_hist = nullptr, *d_acc = nullptr, *d_F = nullptr;
+  auto release = [&]() {
+    for (void* p : {d_q, d_ie, d_ser, d_hist, d_acc, d_F})
+      if (p) (void)hipFree(p);
+  };
+  if (hipMalloc(&d_q, sizeof(long long) * nmon) != hipSuccess || hipMalloc(&d_ie, sizeof(float) * nmon) != hipSuccess ||
+      hipMalloc(&d_ser, sizeof(float) * (size_t)nmon * nsteps) != hipSuccess ||
+      hipMalloc(&d_hist, hist_bytes) != hipSuccess || hipMalloc(&d_acc, sizeof(double) * nd) != hipSuccess ||
+      hipMalloc(&d_F, sizeof(double) * 2 * nmon) != hipSuccess) {
+    (void)hipGetLastError();
+    release();
+    return fail(DTM_ERR_NOMEM, "adjoint temporaries (design-box history " + std::to_string(hist_bytes >> 20) + " MiB)");
+  }
+  auto* hq = static_cast<long long*>(d_q);
+  auto* hie = static_cast<float*>(d_ie);
+  auto* ser = static_cast<float*>(d_ser);
+  auto* hist = static_cast<float*>(d_hist);
+  auto* acc = static_cast<double*>(d_acc);
+  auto* Fd = static_cast<double*>(d_F);
+  const size_t snap = 3 * (size_t)nd;
+  int rc = 0;
+  do {
+    if (hipMemcpyAsync(hq, q.data(), sizeof(long long) * nmon, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(hie, ie.data(), sizeof(float) * nmon, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemsetAsync(acc, 0, sizeof(double) * nd, s->stream) != hipSuccess) {
+      rc = fail(DTM_ERR_DEVICE, "adjoint set-up copies");
+      break;
+    }
+    // forward: record Ez at the monitors and the design-box E every step
+    launch_adj_snap(s->g, fields_ptr(s), mt, hist, s->stream);
+    for (int n = 0; n < nsteps && !rc; n++) {
+      rc = one_step(s);
+      launch_adj_monitor(fields_ptr(s), hq, nmon, ser + (size_t)n * nmon, s->stream);
+      launch_adj_snap(s->g, fields_ptr(s), mt, hist + (size_t)(n + 1) * snap, s->stream);
+    }
+    if (rc) break;
+    std::vector<float> hs((size_t)nmon * nsteps);
+    if (hipMemcpyAsync(hs.data(), ser, sizeof(float) * hs.size(), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
+      rc = fail(DTM_ERR_DEVICE, "adjoint series copy");
+      break;
+    }
+    // F_m = sum_n Ez_m^{n+1} e^{i w (n+1) dt} dt, J = sum_m |F_m|^2
+    std::vector<double> F(2 * (size_t)nmon, 0.0);
+    for (int n = 0; n < nsteps; n++) {
+      const double t = (double)(n + 1) * c.dt;
+      const double cs = cos(omega * t) * c.dt, sn = sin(omega * t) * c.dt;
+      for (int m = 0; m < nmon; m++) {
+        F[2 * m] += hs[(size_t)n * nmon + m] * cs;
+        F[2 * m + 1] += hs[(size_t)n * nmon + m] * sn;
+      }
+    }
+    double J = 0.0;
+    for (int m = 0; m < nmon; m++) J += F[2 * m] * F[2 * m] + F[2 * m + 1] * F[2 * m + 1];
+    *J_out = J;
+    if (hipMemcpyAsync(Fd, F.data(), sizeof(double) * 2 * nmon, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+      rc = fail(DTM_ERR_DEVICE, "adjoint F copy");
+      break;
+    }
+    // adjoint: zeta = M lambda_E, eta = -lambda_H stepped backward by the
+    // forward sweeps; zeta^N = M dJ/dE^N
+    if ((rc = zero_state(s))) break;
+    s->adjoint = true;
+    launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, nsteps, s->stream);
+    for (int n = nsteps - 1; n >= 0 && !rc; n--) {
+      launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)n * snap, hist + (size_t)(n + 1) * snap, acc, s->stream);
+      rc = one_step(s);
+      if (n >= 1) launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
+    }
+    s->adjoint = false;
+    if (rc) break;
+    if (hipMemcpyAsync(grad_inv_eps, acc, sizeof(double) * nd, hipMemcpyDeviceToHost, s->stream) != hipSuccess) {
+      rc = fail(DTM_ERR_DEVICE, "adjoint gradient copy");
+      break;
+    }
+  } while (false);
+  s->adjoint = false;
+  // back to step 0 with zero fields
+  if (!rc) rc = zero_state(s);
+  s->step = 0;
+  s->tab_n0 = -1;
+  if (hipStreamSynchronize(s->stream) != hipSuccess && !rc) rc = fail(DTM_ERR_DEVICE, "adjoint sync");
+  release();
+  return rc;
+}
 
 int dtm_field_read(dtm_solver* s, int comp, float* buf) {
   if (!s || !buf || comp < 0 || comp > 5) return fail(DTM_ERR_ARG, "bad field read arguments");
@@ -1083,10 +1272,13 @@ long long dtm_group_cells(const dtm_solver* s, int g) {
   auto cells = [](const Region& r) {
     return (long long)std::max(0, r.i1 - r.i0) * std::max(0, r.j1 - r.j0) * std::max(0, r.k1 - r.k0);
   };
-  if (g == 0 || g == 2) return cells(s->main);
+  const Region& m = s->torre ? s->main : s->main2;
+  const Region* e = s->torre ? s->edge : s->edge2;
+  const int ne = s->torre ? s->nedge : s->nedge2;
+  if (g == 0 || g == 2) return cells(m);
   if (g == 1 || g == 3) {
     long long t = 0;
-    for (int q = 0; q < s->nedge; q++) t += cells(s->edge[q]);
+    for (int q = 0; q < ne; q++) t += cells(e[q]);
     return t;
   }
   return 0;

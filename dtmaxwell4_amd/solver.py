@@ -63,6 +63,7 @@ class Solver:
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         ni, nj, nk = eps.shape
         abi.check(self.L.dtm_set_design_eps(self.h, i0, j0, k0, ni, nj, nk, _ptr(eps)), "dtm_set_design_eps")
+        self._design_shape = (ni, nj, nk)
 
     def add_sensor(self, fld, x, y, z):
         r = abi.check(self.L.dtm_add_sensor(self.h, fld.encode(), x, y, z), "dtm_add_sensor")
@@ -144,3 +145,17 @@ class Solver:
         v = C.c_double()
         abi.check(self.L.dtm_energy(self.h, C.byref(v)), "dtm_energy")
         return v.value
+
+    def design_gradient(self, nsteps, omega, monitors):
+        """Adjoint gradient of J = sum_m |DFT_omega Ez_m|^2 over nsteps with
+        respect to the design box's 1/eps.  monitors: (i, j, k) cells.
+        Returns (J, dJ/d(1/eps) as an array shaped like the design box).
+        The solver is left at step 0 with zero fields."""
+        mon = np.ascontiguousarray(np.asarray(monitors, dtype=np.int32).reshape(-1, 3))
+        shape = self._design_shape
+        grad = np.zeros(shape, dtype=np.float64)
+        J = C.c_double()
+        abi.check(self.L.dtm_adjoint_design_gradient(self.h, int(nsteps), float(omega), mon.shape[0],
+                                                      mon.ctypes.data, C.byref(J), grad.ctypes.data),
+                  "dtm_adjoint_design_gradient")
+        return J.value, grad

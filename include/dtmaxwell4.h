@@ -230,6 +230,21 @@ int dtm_field_write(dtm_solver* s, int comp, const float* buf);
 /* Discrete field energy sum(eps E^2 + H^2)*dV (double). */
 int dtm_energy(dtm_solver* s, double* out);
 
+/* -------------------------------------------------------------- adjoint */
+/* Gradient of J = sum_m |F_m|^2, F_m = sum_{n<nsteps} Ez_m^{n+1} e^{i omega (n+1) dt} dt,
+ * with Ez_m the raw Ez of monitor cell m (mon_ijk: nmon (i,j,k) triples),
+ * with respect to the design box's inverse permittivity (dtm_set_design_eps).
+ * Runs nsteps forward from step 0 recording the design-box E, then the
+ * discrete adjoint backward (the transposed leapfrog step, which is the
+ * forward step on (eps^-1 lambda_E, -lambda_H)).  J_out gets J;
+ * grad_inv_eps gets dJ/d(1/eps) per design cell (C order, ni*nj*nk doubles).
+ * Afterwards the solver is back at step 0 with zero fields.  Needs FP32
+ * storage and no dispersive or gyrotropic cells; holds (nsteps+1)*3*ni*nj*nk
+ * floats of device memory for the call.  No counterpart in the reference
+ * (it has no adjoint); the explicit API's dtm_adjoint entry. */
+int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmon, const int* mon_ijk,
+                                double* J_out, double* grad_inv_eps);
+
 /* ---------------------------------------------------------------- timing */
 /* Time `nsteps` steps with HIP events on the solver's stream; returns ms. */
 int dtm_time_steps(dtm_solver* s, int nsteps, double* ms);

@@ -19,8 +19,9 @@
  *     periodic eigenmode, the leapfrog energy invariant, div E = 0, and the
  *     6/7 stability bound (DTmxw.cu:332).
  *   - boundaries: zero padding outside the grid (genAlg.py:38 "zerov",
- *     defs.h:40 isOutS) or periodic wrap (defs.h:204).  Pinned by mirror
- *     symmetry and the periodic eigenmode.
+ *     defs.h:40 isOutS) or periodic wrap (defs.h:204).  Pinned by the
+ *     periodic eigenmode (wrap) and the leapfrog invariant in a zero-padded
+ *     box (conservative only if the padding is the exact transpose pair).
  *   - absorbing layer: CPML with the reference's sigma grading
  *     (DTmxw.cu:220-249: sigma_max = V_max ln(1/R) (m+1)/(2L), (1-i/N)^m).
  *     Pinned by the sigma closed form and a reflection bound vs a large box.
@@ -28,17 +29,23 @@
  *     dipole incident field signal.hpp:51-97, plane wave :14-50.  Pinned by
  *     zero leakage outside the box without a scatterer and the closed-form
  *     dipole field identities (far field E.n = 0, |E| = c|B|).
- *   - Lorentz/Drude ADE, dispersion.cpp:41-99.  Pinned by the coefficient
- *     formulas' derivation identities and the discrete eps(w) of the
- *     recursion (Z-transform closed form).
+ *   - Lorentz/Drude ADE, dispersion.cpp:41-99.  Pinned by the discrete
+ *     eps(w) of the recursion (Z-transform) converging at second order to
+ *     the continuous Lorentz/Drude permittivity.
  *   - sensors, sensor.h:51-83.  Pinned by the interpolation formula on an
  *     analytic field.
  *   - FP64 DFT (new on the device; the reference post-processes with an
  *     external script, calc_purcell.sh:2).  Pinned by the DFT of an
  *     analytic waveform.
  *   - magnetised-plasma ADE (no reference counterpart: Gyration() is
- *     disabled, genAlg.py:303-304).  Pinned by the free-plasma dispersion
- *     relation with cyclotron splitting (tests/test_oracle_gyro.py).
+ *     disabled, genAlg.py:303-304).  Pinned by second-order convergence of
+ *     the current update to dJ/dt + nu J = wp^2 E + wc x J and by the
+ *     losslessness of its rotation part (tests/test_oracle_media.py).
+ *   - CoffStruct::set_eps (defs.h:96-117), kept reference-literal including
+ *     its coffAnisoXZ term; pinned by the Cayley identities where gx = 0 and
+ *     by the closed form of the XZ deviation.
+ *   - Purcell post-processing (purcell.gp:2): image-dipole closed form and
+ *     the rows of the reference's own spectra (tests/golden).
  */
 #include <math.h>
 #include <stdint.h>

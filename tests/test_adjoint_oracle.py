@@ -1,0 +1,30 @@
+"""Pins the adjoint algorithm of dtm_adjoint_design_gradient on the FP64
+oracle: the transposed leapfrog step run as the forward step on
+(lambda_E/eps, -lambda_H) gives dJ/d(1/eps) equal to central finite
+differences of J (CPML on every face, soft source, random design box)."""
+import numpy as np
+
+from adjoint_case import fd_gradient, objective, oracle_adjoint, seeded_voxels, small_case
+
+
+def test_oracle_adjoint_matches_finite_differences():
+    cfg, d0, eps, mons, omega, steps = small_case()
+    J, grad = oracle_adjoint(cfg, d0, eps, mons, omega, steps)
+    assert abs(J / objective(cfg, d0, eps, mons, omega, steps) - 1) < 1e-12
+    vox = seeded_voxels(eps.shape, n=6)
+    gfd = fd_gradient(cfg, d0, eps, mons, omega, steps, vox)
+    ga = np.array([grad[v] for v in vox])
+    assert np.all(np.abs(gfd) > 0)
+    assert np.linalg.norm(ga - gfd) / np.linalg.norm(gfd) < 1e-5
+
+
+def test_oracle_adjoint_sign_error_would_fail():
+    """The check is sensitive: dropping the eps^2 of the gradient sum (the
+    chain from lambda_E = eps zeta and dt curl H = eps dE, a plausible
+    slip) moves the gradient far beyond the tolerance."""
+    cfg, d0, eps, mons, omega, steps = small_case(steps=200)
+    J, grad = oracle_adjoint(cfg, d0, eps, mons, omega, steps)
+    vox = seeded_voxels(eps.shape, n=3)
+    gfd = fd_gradient(cfg, d0, eps, mons, omega, steps, vox)
+    wrong = np.array([grad[v] / eps[v] ** 2 for v in vox])  # eps^2 factor dropped
+    assert np.linalg.norm(wrong - gfd) / np.linalg.norm(gfd) > 1e-2
