@@ -90,6 +90,82 @@ def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
     return res
 
 
+def adjoint_run(steps):
+    """Adjoint/forward time ratio on the inverse-design workload: 400x200x200,
+    12-cell CPML, 64x64x32 design box (eps = 1 + 11 rho, rho ~ U[0,1] seeded,
+    3-cell box smoothed), soft Ez dipole, J = sum |DFT Ez|^2 over a 16x16
+    monitor plane at one frequency.  Forward = plain stepping of the same
+    steps; adjoint = dtm_adjoint_design_gradient (its recorded forward pass
+    plus the backward pass), both wall-clock with the device synchronised."""
+    import numpy as np
+
+    from dtmaxwell4_amd import configs
+    from dtmaxwell4_amd.solver import Solver
+    w = configs.inverse_design(steps=steps)
+    i0, j0, k0, eps = w.design
+    mons = [(p[1], p[2], p[3]) for p in w.probes]
+    omega = float(w.omegas[0])
+    s = Solver(w.cfg)
+    s.set_design_eps(i0, j0, k0, eps)
+    log(f"[bench] adjoint workload {w.cfg.nx}x{w.cfg.ny}x{w.cfg.nz}, design {eps.shape}, {len(mons)} monitors, "
+        f"{steps} steps")
+    s.step(20)
+    s.sync()
+    t = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    fwd = time.perf_counter() - t
+    # back to step 0: a fresh solver with the same set-up
+    s.close()
+    s = Solver(w.cfg)
+    s.set_design_eps(i0, j0, k0, eps)
+    t = time.perf_counter()
+    J, grad = s.design_gradient(steps, omega, mons)
+    adj = time.perf_counter() - t
+    s.close()
+    if not (np.isfinite(J) and np.isfinite(grad).all()):
+        raise RuntimeError("non-finite adjoint result")
+    cells = w.cfg.nx * w.cfg.ny * w.cfg.nz
+    return dict(ratio=adj / fwd, forward_s=fwd, adjoint_call_s=adj, steps=steps,
+                forward_gcells=cells * steps / fwd / 1e9, J=J, grad_norm=float(np.linalg.norm(grad)),
+                workload=f"inverse design {w.cfg.nx}x{w.cfg.ny}x{w.cfg.nz}, design {list(eps.shape)}, "
+                         f"{len(mons)} monitors, {steps} steps (call = recorded forward + backward)")
+
+
+def purcell_run(steps):
+    """Purcell error of FP16 storage on the dispersive dipole workload:
+    384^3, 12-cell CPML, Drude (eps_inf 3.7, wp 1.37e16, gamma 2.7e13 rad/s) +
+    Lorentz sphere r=40, Ez DipoleTFSF 4 cells above the surface, 64 FP64 DFT
+    bins on 0.5-1.5 f0.  Error = max_w |P16 - P32| / max_w |P32| (purcell.gp:2)."""
+    import numpy as np
+
+    from dtmaxwell4_amd import abi, configs, purcell
+    from dtmaxwell4_amd.solver import Solver
+    out = {}
+    curves = {}
+    for name, store in (("fp32", abi.STORE_FP32), ("fp16", abi.STORE_FP16)):
+        w = configs.drude_lorentz_purcell(steps=steps, storage=store)
+        s = configs.apply(w, Solver(w.cfg))
+        t = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        el = time.perf_counter() - t
+        FE = s.dft()[0]
+        s.close()
+        Fd = purcell.dipole_spectrum(w.cfg.tfsf, w.omegas, steps, w.cfg.dt)
+        curves[name] = purcell.purcell_factor(w.omegas, FE, Fd)
+        cells = w.cfg.nx * w.cfg.ny * w.cfg.nz
+        out[name + "_gcells"] = cells * steps / el / 1e9
+        log(f"[bench] purcell {name}: {out[name + '_gcells']:.2f} Gcells/s, P in "
+            f"[{curves[name].min():.4g}, {curves[name].max():.4g}]")
+    P32, P16 = curves["fp32"], curves["fp16"]
+    out["fp16_vs_fp32_error"] = float(np.max(np.abs(P16 - P32)) / np.max(np.abs(P32)))
+    out["P_fp32_range"] = [float(P32.min()), float(P32.max())]
+    out["workload"] = (f"Drude+Lorentz sphere r=40 in 384^3, dipole TF/SF 4 cells from the surface, "
+                       f"{len(P32)} DFT bins, {steps} steps")
+    return out
+
+
 def cpu_baseline(nx, ny, nz, budget_s=15.0):
     """FP32 OpenMP restatement of the same equations (oracle/liboracle32.so) on
     a bounded sample: the full grid for as many steps as fit ~budget_s."""
@@ -123,6 +199,10 @@ def main():
     ap.add_argument("--sweep", choices=("twopass", "fused"), default="twopass",
                     help="time-step schedule of the headline value; the other one is reported beside it")
     ap.add_argument("--no-alt", action="store_true", help="skip the other schedule")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio run")
+    ap.add_argument("--adjoint-steps", type=int, default=6000)
+    ap.add_argument("--no-purcell", action="store_true", help="skip the FP16 Purcell-error run")
+    ap.add_argument("--purcell-steps", type=int, default=4000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the benchmark, run a plain device copy of six grid-sized FP32 arrays "
                          "(the scale reference for HBM byte counters)")
@@ -144,6 +224,15 @@ def main():
     if not args.no_fp16:
         r16 = gpu_run(abi.STORE_FP16, sw, args.steps, args.warmup, nx, ny, nz)
         log(f"[bench] fp16 {r16['gcells']:.2f} Gcells/s  groups={json.dumps(r16['groups'])}")
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_run(args.adjoint_steps)
+        log(f"[bench] adjoint/forward {adj['ratio']:.3f} (forward {adj['forward_s']:.3f} s, "
+            f"adjoint call {adj['adjoint_call_s']:.3f} s)")
+    pur = None
+    if not args.no_purcell:
+        pur = purcell_run(args.purcell_steps)
+        log(f"[bench] purcell fp16 vs fp32 error {pur['fp16_vs_fp32_error']:.3e}")
     cpu = None
     if not args.no_cpu:
         try:
@@ -177,11 +266,14 @@ def main():
         "effective_yee_frac": round(r32["gcells"] * 72 / HBM_PEAK_GBS, 4),
         "fp16_store": None if r16 is None else {
             "value": round(r16["gcells"], 3), "unit": "Gcells/s", "speedup_vs_fp32": round(r16["gcells"] / r32["gcells"], 3),
-            "dominant_kernel_gbs": r16["groups"][r16["dominant"]]["gbs"]},
+            "dominant_kernel_gbs": r16["groups"][r16["dominant"]]["gbs"],
+            "purcell_error": None if pur is None else pur["fp16_vs_fp32_error"]},
+        "purcell": pur,
         "other_schedule_fp32": None if ralt is None else {
             "kernel": ralt["kernel"], "value": round(ralt["gcells"], 3), "unit": "Gcells/s",
             "groups": ralt["groups"]},
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": None if adj is None else round(adj["ratio"], 3),
+        "adjoint": adj,
         "cpu_baseline": cpu,
         "kernel_groups_fp32": r32["groups"],
     }

@@ -12,6 +12,7 @@ COMP_NAMES = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 BC_ZERO, BC_PERIODIC = 0, 1
 STORE_FP32, STORE_FP16 = 0, 1
 SWEEP_TWOPASS, SWEEP_FUSED = 0, 1
+PML_CPML, PML_SPLIT = 0, 1
 TFSF_NONE, TFSF_DIPOLE, TFSF_PLANE = 0, 1, 2
 WAVE_RAISED_COS, WAVE_GAUSS_SINE = 0, 1
 MAX_MATS, MAX_POLES, MAX_SOFT = 64, 4, 16

@@ -44,7 +44,12 @@ enum { DTM_TFSF_NONE = 0, DTM_TFSF_DIPOLE = 1, DTM_TFSF_PLANE = 2 };
  * GAUSS_SINE is exp(-((tau-t0)/tw)^2) sin(w tau) with tau=phase/w
  * (the envelope of signal.hpp:55-57).                                     */
 enum { DTM_WAVE_RAISED_COS = 0, DTM_WAVE_GAUSS_SINE = 1 };
-enum { DTM_PML_CPML = 0 };
+/* Absorbing layer: CPML (psi only in the slabs, kappa = 1, alpha = 0), or the
+ * reference's split-field layer (genAlg.py:117-148): in the slabs each
+ * component is the sum of one part per derivative axis, each part advanced
+ * as part = k1 part +- k2 dif with k2 = 1/(1 + sigma dt/2), k1 = 2 k2 - 1
+ * (DTmxw.cu:250-265).  Both use the same sigma grading. */
+enum { DTM_PML_CPML = 0, DTM_PML_SPLIT = 1 };
 enum {
   DTM_OK = 0,
   DTM_ERR_ARG = -1,      /* invalid argument / configuration */
@@ -158,7 +163,7 @@ typedef struct dtm_config {
   double pml_order;       /* grading exponent m (reference: 4, DTmxw.cu:224) */
   double pml_R;           /* target reflection (reference: 1e-4, DTmxw.cu:225) */
   double pml_vmax;        /* V_max (reference: shotpoint.V_max = c) */
-  int pml_kind;           /* DTM_PML_CPML */
+  int pml_kind;           /* DTM_PML_CPML or DTM_PML_SPLIT */
   int storage;            /* DTM_STORE_FP32 / DTM_STORE_FP16 */
   int scale_e_log2[3];    /* FP16 storage: E_c stored as E_c * 2^s */
   int scale_h_log2[3];

@@ -82,6 +82,13 @@ typedef struct orc_sim {
   double* pml_b[3];
   double* pml_c[3];
   int cpml_on;
+  /* split-field layer (pml_kind DTM_PML_SPLIT): k1, k2 per half-coordinate,
+   * and the part of each component along its second derivative axis */
+  int split_on;
+  double* sk1[3];
+  double* sk2[3];
+  real* spE[3];
+  real* spH[3];
   uint8_t* mat[3];
   double* design_eps; /* per cell, 0 = not overridden */
   /* ADE state (per E component, per cell) */
@@ -293,13 +300,26 @@ static void build_cpml(orc_sim* s) {
   const dtm_config* c = &s->cfg;
   int nn[3] = {s->nx, s->ny, s->nz};
   double dd[3] = {c->dx, c->dy, c->dz};
+  const int split = c->pml_kind == DTM_PML_SPLIT;
   s->cpml_on = 0;
+  s->split_on = 0;
   for (int a = 0; a < 3; a++) {
     s->pml_b[a] = (double*)calloc(2 * nn[a] + 1, sizeof(double));
     s->pml_c[a] = (double*)calloc(2 * nn[a] + 1, sizeof(double));
+    s->sk1[a] = (double*)calloc(2 * nn[a] + 1, sizeof(double));
+    s->sk2[a] = (double*)calloc(2 * nn[a] + 1, sizeof(double));
     for (int h = 0; h <= 2 * nn[a]; h++) {
       double sg = orc_pml_sigma(h, nn[a], c->npml[2 * a], c->npml[2 * a + 1], dd[a], c->pml_order,
                                 c->pml_R, c->pml_vmax);
+      /* split-field coefficients, setPMLcoeffs (DTmxw.cu:250-265) */
+      s->sk2[a][h] = 1.0 / (1.0 + 0.5 * c->dt * sg);
+      s->sk1[a][h] = 2.0 * s->sk2[a][h] - 1.0;
+      if (split) {
+        s->pml_b[a][h] = 1.0;
+        s->pml_c[a][h] = 0.0;
+        if (sg > 0) s->split_on = 1;
+        continue;
+      }
       double b = exp(-sg * c->dt);
       s->pml_b[a][h] = b;
       s->pml_c[a][h] = b - 1.0;
@@ -443,6 +463,11 @@ orc_sim* orc_create(const dtm_config* cfg) {
     s->mat[c] = (uint8_t*)xcalloc(s->n, 1);
   }
   build_cpml(s);
+  if (s->split_on)
+    for (int c = 0; c < 3; c++) {
+      s->spE[c] = (real*)xcalloc(s->n, sizeof(real));
+      s->spH[c] = (real*)xcalloc(s->n, sizeof(real));
+    }
   if (s->cpml_on) {
     for (int c = 0; c < 3; c++)
       for (int a = 0; a < 3; a++)
@@ -502,6 +527,10 @@ void orc_destroy(orc_sim* s) {
     }
     free(s->pml_b[c]);
     free(s->pml_c[c]);
+    free(s->sk1[c]);
+    free(s->sk2[c]);
+    free(s->spE[c]);
+    free(s->spH[c]);
   }
   free(s->design_eps);
   free(s->series);
@@ -632,6 +661,7 @@ static int in_ext(double pos, double lo, double hi, double d) { return pos >= lo
  * crosses the box's z faces is still fast; the cells near those faces take
  * the general path (tfsf_zcell). */
 static int row_fast(const orc_sim* s, int i, int j, int tf) {
+  if (s->split_on) return 0; /* the split-field layer is on the general path only */
   if (s->cfg.bc[0] == DTM_BC_PERIODIC || s->cfg.bc[1] == DTM_BC_PERIODIC || s->cfg.bc[2] == DTM_BC_PERIODIC)
     return 0;
   if (i < 2 || i > s->nx - 3 || j < 2 || j > s->ny - 3 || s->nz < 5) return 0;
@@ -727,6 +757,21 @@ static void h_cell_general(orc_sim* s, int i, int j, int k, int tf, double tinc)
         d2 += ps[q];
       }
     }
+    if (s->split_on) {
+      /* genAlg.py:117-148: H = P1 + P2, P_a = k1_a P_a -+ k2_a dt dif_a */
+      int h[3];
+      halfc(comp, i, j, k, h);
+      double k11 = s->sk1[a1][h[a1]], k21 = s->sk2[a1][h[a1]];
+      double k12 = s->sk1[a2][h[a2]], k22 = s->sk2[a2][h[a2]];
+      if (k21 != 1.0 || k22 != 1.0) {
+        double p2 = s->spH[c][q], p1 = s->H[c][q] - p2;
+        p1 = k11 * p1 - k21 * s->cfg.dt * d1;
+        p2 = k12 * p2 + k22 * s->cfg.dt * d2;
+        s->spH[c][q] = (real)p2;
+        s->H[c][q] = (real)(p1 + p2);
+        continue;
+      }
+    }
     s->H[c][q] = (real)(s->H[c][q] - s->cfg.dt * (d1 - d2));
   }
 }
@@ -753,7 +798,7 @@ static void e_cell_general(orc_sim* s, int i, int j, int k, int tf, double tinc,
   const dtm_config* cfg = &s->cfg;
   double dt = cfg->dt;
   size_t q = IDX(s, i, j, k);
-  double curl[3];
+  double curl[3], dd1[3], dd2[3];
   for (int c = 0; c < 3; c++) {
     int a1 = CURL[c][0], b1 = CURL[c][1], a2 = CURL[c][2], b2 = CURL[c][3];
     double d1 = deriv(s, s->H[b1], c, i, j, k, a1, tf, tinc);
@@ -774,9 +819,14 @@ static void e_cell_general(orc_sim* s, int i, int j, int k, int tf, double tinc,
       }
     }
     curl[c] = d1 - d2;
+    dd1[c] = d1;
+    dd2[c] = d2;
     for (int so = 0; so < cfg->nsoft; so++) {
       const dtm_soft_source* ss = &cfg->soft[so];
-      if (ss->comp == c && ss->i == i && ss->j == j && ss->k == k) curl[c] -= Jsrc[so];
+      if (ss->comp == c && ss->i == i && ss->j == j && ss->k == k) {
+        curl[c] -= Jsrc[so];
+        dd1[c] -= Jsrc[so];
+      }
     }
   }
   /* gyrotropic current: J^{n+1/2} from J^{n-1/2} and E^n (co-located) */
@@ -789,7 +839,10 @@ static void e_cell_general(orc_sim* s, int i, int j, int k, int tf, double tinc,
       for (int t = 0; t < 3; t++) v += s->gyA[mg][3 * r + t] * J0[t] + s->gyB[mg][3 * r + t] * E0[t];
       s->Jg[r][q] = (real)v;
     }
-    for (int r = 0; r < 3; r++) curl[r] -= s->Jg[r][q];
+    for (int r = 0; r < 3; r++) {
+      curl[r] -= s->Jg[r][q];
+      dd1[r] -= s->Jg[r][q];
+    }
   }
   for (int c = 0; c < 3; c++) {
     int m;
@@ -813,6 +866,22 @@ static void e_cell_general(orc_sim* s, int i, int j, int k, int tf, double tinc,
       s->Em[c][q] = (real)Ep;
       s->E[c][q] = (real)En;
     } else {
+      if (s->split_on) {
+        /* genAlg.py:117-148 on E: E = P1 + P2, P_a = k1_a P_a +- k2_a dt/eps dif_a */
+        int h[3];
+        int a1 = CURL[c][0], a2 = CURL[c][2];
+        halfc(c, i, j, k, h);
+        double k11 = s->sk1[a1][h[a1]], k21 = s->sk2[a1][h[a1]];
+        double k12 = s->sk1[a2][h[a2]], k22 = s->sk2[a2][h[a2]];
+        if (k21 != 1.0 || k22 != 1.0) {
+          double p2 = s->spE[c][q], p1 = s->E[c][q] - p2;
+          p1 = k11 * p1 + k21 * dt * ie * dd1[c];
+          p2 = k12 * p2 - k22 * dt * ie * dd2[c];
+          s->spE[c][q] = (real)p2;
+          s->E[c][q] = (real)(p1 + p2);
+          continue;
+        }
+      }
       s->E[c][q] = (real)(s->E[c][q] + dt * ie * curl[c]);
     }
   }

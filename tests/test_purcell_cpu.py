@@ -1,0 +1,39 @@
+"""The package's Purcell post-processing (dtmaxwell4_amd/purcell.py) against
+the oracle's literal purcell.gp:2 expression and waveform, and the spectrum
+helper against a direct sum."""
+import numpy as np
+import pytest
+
+from dtmaxwell4_amd import abi, purcell
+from oracle import oracle as orc
+
+
+def test_purcell_factor_is_the_reference_expression():
+    rng = np.random.RandomState(1)
+    for _ in range(50):
+        w = rng.uniform(0.5, 20)
+        FE = complex(*rng.normal(size=2)) * 10 ** rng.uniform(-3, 3)
+        Fd = complex(*rng.normal(size=2)) * 10 ** rng.uniform(-3, 3)
+        ref = orc.purcell(w, abs(FE), np.angle(FE), abs(Fd), np.angle(Fd))
+        assert purcell.purcell_factor(w, FE, Fd) == pytest.approx(ref, rel=1e-12, abs=1e-12)
+
+
+@pytest.mark.parametrize("kind", [abi.WAVE_RAISED_COS, abi.WAVE_GAUSS_SINE])
+def test_waveform_matches_oracle(kind):
+    w, kEnv, t0, tw = 7.0, 0.2, 1.3, 0.4
+    t = np.linspace(-0.5, 6.0, 97)
+    ours = purcell.waveform(kind, w, kEnv, t0, tw, t)
+    ref = np.array([orc.wave(kind, w, kEnv, t0, tw, x) for x in t])
+    np.testing.assert_allclose(ours, ref, rtol=1e-12, atol=1e-15)
+
+
+def test_dipole_spectrum_direct_sum():
+    t = abi.Tfsf()
+    t.waveform, t.w, t.t0, t.tw, t.amp = abi.WAVE_GAUSS_SINE, 5.0, 1.0, 0.3, 2.0
+    dt, steps = 0.01, 400
+    om = np.array([4.0, 5.0, 6.5])
+    F = purcell.dipole_spectrum(t, om, steps, dt)
+    for q, wq in enumerate(om):
+        ref = sum(2.0 * orc.wave(t.waveform, t.w, 0.0, t.t0, t.tw, (n + 1) * dt) * np.exp(1j * wq * (n + 1) * dt) * dt
+                  for n in range(steps))
+        assert F[q] == pytest.approx(ref, rel=1e-12)
