@@ -32,6 +32,7 @@ class Solver:
         self.h = h
         self.nprobe = 0
         self.nfreq = 0
+        self._design_shape = (0, 0, 0)  # no design box yet
 
     def close(self):
         if getattr(self, "h", None):

@@ -28,7 +28,7 @@ def test_adjoint_gradient_matches_finite_differences(sweep):
     print("adjoint vs FD rel err", err)
     assert err < 1e-3
     # the call leaves the solver at step 0 with zero fields
-    assert s.time_step() == 0
+    assert s.time_step == 0
     assert all(np.abs(s.field(c)).max() == 0 for c in range(6))
     # and a second call reproduces the first
     J2, grad2 = s.design_gradient(steps, omega, mons)
