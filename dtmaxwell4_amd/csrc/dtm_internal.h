@@ -22,8 +22,13 @@ struct Grid {
   int per[3];        // periodic axes
   float rd[3];       // 1/(24 d_a): the (2,4) weights 27/24, -1/24 (defs.h:3-13)
   float dt;
-  const void* zero;  // 256 zero bytes: the load target of positions outside the grid
+  const void* zero;  // 256 zero bytes
+  unsigned zidx;     // = nx*ny*nz: index of the zero pad that ends every field and class array
 };
+
+// Every field array and the class array hold kPad zero elements after the
+// last cell; loads of positions outside the grid read index nx*ny*nz.
+constexpr int kPad = 64;
 
 // CPML: psi lives only in the slabs.  For axis a the two affected
 // components are (a+1)%3 (slot 0) and (a+2)%3 (slot 1).
@@ -36,6 +41,12 @@ struct Cpml {
   const float* cH[3];
   float* psiE[3][2];
   float* psiH[3][2];
+  // split-field layer: b* hold k1 = 2 k2 - 1 and c* hold k2 = 1/(1 + sigma dt/2)
+  // (1 and 1 outside the slab); p2* hold, per component and cell, the part
+  // along the component's second curl axis (the first is field - p2)
+  int split;
+  float* p2E[3];
+  float* p2H[3];
 };
 
 // Material class table: class byte per cell -> per-component inverse eps and

@@ -190,6 +190,32 @@ __device__ __forceinline__ long long psi_index(const Grid& g, const Cpml& p, int
   return ((long long)i * g.ny + j) * P + s;
 }
 
+// Split-field part update (genAlg.py:117-148): F = P1 + P2 with P2 stored,
+// P1 <- k1a P1 + k2a ua and P2 <- k1b P2 + k2b ub, ua / ub the increments of
+// the first and second curl term.  Outside every slab of the component
+// (k2 = 1 on both axes) it is the plain update F + ua + ub.
+__device__ __forceinline__ float split_step(float* P2, long long q, float f0, float k1a, float k2a, float ua,
+                                            float k1b, float k2b, float ub) {
+  if (k2a == 1.0f && k2b == 1.0f) return f0 + ua + ub;
+  const float p2 = P2[q], p1 = f0 - p2;
+  const float n1 = k1a * p1 + k2a * ua, n2 = k1b * p2 + k2b * ub;
+  P2[q] = n2;
+  return n1 + n2;
+}
+
+// 32-bit psi indices of the sweeps (psi arrays are far below 2^32 floats):
+// z slab: base + slab index; x and y slabs: base + (k - k_lane0).
+__device__ __forceinline__ unsigned psi_base_z(const Grid& g, const Cpml& p, int i, int j) {
+  return ((unsigned)i * (unsigned)g.ny + (unsigned)j) * (unsigned)(p.plo[2] + p.phi[2]);
+}
+__device__ __forceinline__ unsigned psi_base_y(const Grid& g, const Cpml& p, int ys, int i, int k) {
+  return ys < 0 ? 0u : ((unsigned)i * (unsigned)(p.plo[1] + p.phi[1]) + (unsigned)ys) * (unsigned)g.nz + (unsigned)k;
+}
+__device__ __forceinline__ unsigned psi_base_x(const Grid& g, const Cpml& p, int xs, int j, int k) {
+  (void)p;
+  return ((unsigned)xs * (unsigned)g.ny + (unsigned)j) * (unsigned)g.nz + (unsigned)k;
+}
+
 // psi <- b psi + c D ; D <- D + psi   (CPML, kappa=1, alpha=0)
 __device__ __forceinline__ void cpml_apply(float* psi, long long pq, float b, float c, float& D) {
   float v = b * psi[pq] + c * D;
@@ -219,17 +245,19 @@ struct MarchSet {
   int nbk[6], nbj[6], nbs[6];
   int start[7];
   int total, per, seg;
+  int bz;  // z extent of a tile: BZ * (cells per lane)
 };
 
-static MarchSet make_march(const Region* regs, int nreg, int seg) {
+static MarchSet make_march(const Region* regs, int nreg, int seg, int bz) {
   MarchSet s{};
   s.seg = seg;
+  s.bz = bz;
   int acc = 0, m = 0;
   for (int t = 0; t < nreg && m < 6; t++) {
     const Region& r = regs[t];
     if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
     s.r[m] = r;
-    s.nbk[m] = (r.k1 - r.k0 + BZ - 1) / BZ;
+    s.nbk[m] = (r.k1 - r.k0 + bz - 1) / bz;
     s.nbj[m] = (r.j1 - r.j0 + MBY - 1) / MBY;
     s.nbs[m] = (r.i1 - r.i0 + seg - 1) / seg;
     s.start[m] = acc;
@@ -260,7 +288,7 @@ __device__ __forceinline__ bool march_locate(const MarchSet& ms, MarchTile& t) {
   const int bs = loc / ms.nbj[r];
   const Region& R = ms.r[r];
   t.r = r;
-  t.k0 = R.k0 + bk * BZ;
+  t.k0 = R.k0 + bk * ms.bz;
   t.j0 = R.j0 + bj * MBY;
   t.i0 = R.i0 + bs * ms.seg;
   t.i1 = min(t.i0 + ms.seg, R.i1);
@@ -305,50 +333,91 @@ __device__ __forceinline__ T ldr(const T* F, long long base, long long off) {
   return (base < 0 || off < 0) ? rzero<T>() : F[base + off];
 }
 // Address of a position under the boundary rule: outside the grid (base or
-// off < 0) it is the zero page, so the load itself is unconditional and its
-// destination register never needs a merge (a merge would force a wait for
-// the load before the next plane).
+// off < 0) it is the array's zero pad, so the load itself is unconditional
+// and its destination register never needs a merge (a merge would force a
+// wait for the load before the next plane).  The index is 32-bit (grids up
+// to 2^32 - kPad cells), which keeps address arithmetic out of 64-bit
+// registers.
 template <typename T>
-__device__ __forceinline__ const T* at(const T* F, long long base, long long off, const void* zero) {
-  return (base < 0 || off < 0) ? static_cast<const T*>(zero) : F + base + off;
+__device__ __forceinline__ const T* at(const T* F, long long base, long long off, unsigned zidx) {
+  const unsigned idx = (base < 0 || off < 0) ? zidx : (unsigned)(base + off);
+  return F + idx;
 }
 
 // Ring slot of plane p in a 5-plane LDS ring (p >= -3).
 __device__ __forceinline__ int ring5(int p) { return (p + 10) % 5; }
 
 // ================================================================ H sweep
-// Operands of one plane, loaded two planes ahead: the Ex tile values of
-// plane p (own, y-halo row, z-halo column), the H read-modify-write
-// operands of plane p, and the Ey / Ez tile values of plane p+1 (the x
-// stencil of H reaches one plane ahead).
-template <typename T> struct PreH {
-  T ex, exy, exz, hx, hy, hz, ey, eyz, ez, ezy;
+// V consecutive z cells per lane (V = 2 for FP16 storage on an even nz: the
+// lane loads one __half2, so a prefetch register carries two cells, the
+// same bytes in flight per register as FP32).  Raw word of V cells:
+template <int STORE, int V> struct Vec;
+template <> struct Vec<0, 1> { using P = float; };
+template <> struct Vec<1, 1> { using P = __half; };
+template <> struct Vec<1, 2> { using P = __half2; };
+__device__ __forceinline__ void unpackv(float p, float, float* o) { o[0] = p; }
+__device__ __forceinline__ void unpackv(__half p, float inv, float* o) { o[0] = __half2float(p) * inv; }
+__device__ __forceinline__ void unpackv(__half2 p, float inv, float* o) {
+  const float2 f = __half22float2(p);
+  o[0] = f.x * inv;
+  o[1] = f.y * inv;
+}
+template <typename P> __device__ __forceinline__ P pzero();
+template <> __device__ __forceinline__ float pzero<float>() { return 0.0f; }
+template <> __device__ __forceinline__ __half pzero<__half>() { return __ushort_as_half((unsigned short)0); }
+template <> __device__ __forceinline__ __half2 pzero<__half2>() { return __half2(pzero<__half>(), pzero<__half>()); }
+// store V values at element q (q even for V = 2)
+__device__ __forceinline__ void stv(float* F, long long q, const float* v, float) { F[q] = v[0]; }
+__device__ __forceinline__ void stv(__half* F, long long q, const float* v, float s) {
+  F[q] = __float2half_rn(v[0] * s);
+}
+__device__ __forceinline__ void stv2(__half* F, long long q, const float* v, float s) {
+  *reinterpret_cast<__half2*>(F + q) = __floats2half2_rn(v[0] * s, v[1] * s);
+}
+// word of V cells at a position under the boundary rule (see at())
+template <typename P, typename T>
+__device__ __forceinline__ P ldv(const T* F, long long base, long long off, unsigned zidx) {
+  const unsigned idx = (base < 0 || off < 0) ? zidx : (unsigned)(base + off);
+  return *reinterpret_cast<const P*>(F + idx);
+}
+
+// Operands of one plane, loaded PF planes ahead: the Ex tile words of
+// plane p (own, y-halo row) and the Ex z-halo value, the H read-modify-write
+// words of plane p, and the Ey / Ez tile values of plane p+1 (the x stencil
+// of H reaches one plane ahead).
+template <typename P, typename T> struct PreH {
+  P ex, exy, hx, hy, hz, ey, ez, ezy;
+  T exz, eyz;
 };
 
 // GEN bit 0: x boundary rule and x CPML (per plane, block-uniform);
 // bit 1: y CPML (per row).  The y and z boundary rules and the z CPML are
 // always on: they are per-thread constants.
-template <int STORE, int GEN>
-__global__ __launch_bounds__(BZ* MBY) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
+// SP: split-field absorbing layer instead of the CPML (one cell per lane).
+template <int STORE, int GEN, int V, bool SP = false>
+__global__ __launch_bounds__(BZ* MBY, 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
   using T = typename Raw<STORE>::T;
+  using P = typename Vec<STORE, V>::P;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
-  __shared__ float sEx[2][MBY + 3][MLZ];  // [row j-j0+2][col k-k0+2], plane i
-  __shared__ float rEy[5][MBY][MLZ];      // [row j-j0][col k-k0+2], ring of planes
-  __shared__ float rEz[5][MBY + 3][BZ];   // [row j-j0+2][col k-k0], ring of planes
-  __shared__ float sHo[3][MBY][BZ];       // this thread's H operands of plane i
+  constexpr int BZV = BZ * V, LZ = BZV + 3;
+  __shared__ float sEx[2][MBY + 3][LZ];  // [row j-j0+2][col k-k0+2], plane i
+  __shared__ float rEy[5][MBY][LZ];      // [row j-j0][col k-k0+2], ring of planes
+  __shared__ float rEz[5][MBY + 3][BZV]; // [row j-j0+2][col k-k0], ring of planes
+  __shared__ float sHo[3][MBY][BZV];     // this thread's H operands of plane i
   MarchTile t;
   if (!march_locate(ms, t)) return;
   const Region& R = ms.r[t.r];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int j = t.j0 + ty, k = t.k0 + tx;
-  const bool inreg = j < R.j1 && k < R.k1;
+  const int j = t.j0 + ty, k = t.k0 + V * tx;  // first of this lane's V cells
+  const int c0 = V * tx;                       // its tile column
+  const bool inreg = j < R.j1 && k < R.k1;     // V = 2 needs an even nz: cells pair up
   const long long own = plane_off(g, j, k);
-  // halo positions: rows j0-2, j0-1, j0+MBY (waves ty<3); columns k0-2, k0-1, k0+64 (lanes tx<3)
+  // halo positions: rows j0-2, j0-1, j0+MBY (waves ty<3); columns k0-2, k0-1, k0+BZV (lanes tx<3)
   const bool yw = ty < 3, zl = tx < 3;
   const int yrow = ty < 2 ? ty : MBY + 2;
   const long long yh = yw ? plane_off(g, t.j0 + (ty < 2 ? ty - 2 : MBY), k) : -1;
-  const int zcol = tx < 2 ? tx : BZ + 2;
-  const long long zh = zl ? plane_off(g, j, t.k0 + (tx < 2 ? tx - 2 : BZ)) : -1;
+  const int zcol = tx < 2 ? tx : BZV + 2;
+  const long long zh = zl ? plane_off(g, j, t.k0 + (tx < 2 ? tx - 2 : BZV)) : -1;
   const T* Ex = f.E[0];
   const T* Ey = f.E[1];
   const T* Ez = f.E[2];
@@ -357,130 +426,191 @@ __global__ __launch_bounds__(BZ* MBY) void k_update_h(Grid g, Fields<STORE> f, S
   T* Hz = f.H[2];
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
-  const T z0 = rzero<T>();
 
   const long long hq = inreg ? own : -1;  // H operands: owned cells only
-  auto fetch = [&](PreH<T>& S, int p) {
+  auto fetch = [&](PreH<P, T>& S, int p) {
     const long long pb = plane_base<GX>(g, p);
     const long long pn = plane_base<GX>(g, p + 1);
-    S.ex = *at(Ex, pb, own, g.zero);
-    S.ey = *at(Ey, pn, own, g.zero);
-    S.ez = *at(Ez, pn, own, g.zero);
-    S.exy = *at(Ex, pb, yh, g.zero);
-    S.ezy = *at(Ez, pn, yh, g.zero);
-    S.exz = *at(Ex, pb, zh, g.zero);
-    S.eyz = *at(Ey, pn, zh, g.zero);
-    S.hx = *at((const T*)Hx, pb, hq, g.zero);
-    S.hy = *at((const T*)Hy, pb, hq, g.zero);
-    S.hz = *at((const T*)Hz, pb, hq, g.zero);
+    S.ex = ldv<P>(Ex, pb, own, g.zidx);
+    S.ey = ldv<P>(Ey, pn, own, g.zidx);
+    S.ez = ldv<P>(Ez, pn, own, g.zidx);
+    S.exy = ldv<P>(Ex, pb, yh, g.zidx);
+    S.ezy = ldv<P>(Ez, pn, yh, g.zidx);
+    S.exz = *at(Ex, pb, zh, g.zidx);
+    S.eyz = *at(Ey, pn, zh, g.zidx);
+    S.hx = ldv<P>((const T*)Hx, pb, hq, g.zidx);
+    S.hy = ldv<P>((const T*)Hy, pb, hq, g.zidx);
+    S.hz = ldv<P>((const T*)Hz, pb, hq, g.zidx);
   };
   // Ey / Ez tile of plane p straight into its ring slot (prologue only)
   auto ring_direct = [&](int p) {
     const long long pb = plane_base<GX>(g, p);
     const int s = ring5(p);
-    rEy[s][ty][tx + 2] = cvt(ldr(Ey, pb, own), iey);
-    rEz[s][ty + 2][tx] = cvt(ldr(Ez, pb, own), iez);
-    if (yw) rEz[s][yrow][tx] = cvt(ldr(Ez, pb, yh), iez);
+    float v[V];
+    unpackv(ldv<P>(Ey, pb, own, g.zidx), iey, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rEy[s][ty][c0 + q + 2] = v[q];
+    unpackv(ldv<P>(Ez, pb, own, g.zidx), iez, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rEz[s][ty + 2][c0 + q] = v[q];
+    if (yw) {
+      unpackv(ldv<P>(Ez, pb, yh, g.zidx), iez, v);
+#pragma unroll
+      for (int q = 0; q < V; q++) rEz[s][yrow][c0 + q] = v[q];
+    }
     if (zl) rEy[s][ty][zcol] = cvt(ldr(Ey, pb, zh), iey);
   };
 
-  const int zs = pm.on ? slab_index(k, g.nz, pm.plo[2], pm.phi[2]) : -1;
   const int ysl = (GY && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
   const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
-  // CPML coefficients of this column (z and y are fixed per thread)
-  const float bz = zs >= 0 ? pm.bH[2][k] : 0.f, cz = zs >= 0 ? pm.cH[2][k] : 0.f;
+  // CPML state of this lane's cells (z and y are fixed per thread)
+  int zs[V];
+  float bz[V], cz[V];
+#pragma unroll
+  for (int q = 0; q < V; q++) {
+    zs[q] = pm.on ? slab_index(k + q, g.nz, pm.plo[2], pm.phi[2]) : -1;
+    bz[q] = zs[q] >= 0 ? pm.bH[2][k + q] : 0.f;
+    cz[q] = zs[q] >= 0 ? pm.cH[2][k + q] : 0.f;
+  }
   const float by = ysl >= 0 ? pm.bH[1][j] : 0.f, cy = ysl >= 0 ? pm.cH[1][j] : 0.f;
 
-  PreH<T> A{z0, z0, z0, z0, z0, z0, z0, z0, z0, z0}, B = A;
+  // PF prefetch sets, each a plane ahead of the next: three planes in flight
+  // where the registers allow it (the y-CPML variant keeps two).
+  constexpr int PF = (GY || V == 2) ? 2 : 3;
+  PreH<P, T> S[PF];
+#pragma unroll
+  for (int u = 0; u < PF; u++) {
+    S[u].ex = S[u].exy = S[u].hx = S[u].hy = S[u].hz = S[u].ey = S[u].ez = S[u].ezy = pzero<P>();
+    S[u].exz = S[u].eyz = rzero<T>();
+  }
   ring_direct(t.i0 - 2);
   ring_direct(t.i0 - 1);
   ring_direct(t.i0);
   // Prefetches are unconditional (a plane past the segment is clamped to
   // its last plane): with no branch around them the compiler's in-order
-  // wait counts stay exact and a wait never reaches the other set.
-  fetch(A, t.i0);
-  fetch(B, min(t.i0 + 1, t.i1 - 1));
+  // wait counts stay exact and a wait never reaches the other sets.
+#pragma unroll
+  for (int u = 0; u < PF; u++) fetch(S[u], min(t.i0 + u, t.i1 - 1));
 
-  auto step = [&](int i, PreH<T>& S) {
+  auto step = [&](int i, PreH<P, T>& Q) {
     const int b = i & 1;
     const int s0 = ring5(i - 2), s1 = ring5(i - 1), s2 = ring5(i), s3 = ring5(i + 1);
-    // publish plane i (Ex) and plane i+1 (Ey, Ez)
-    sEx[b][ty + 2][tx + 2] = cvt(S.ex, iex);
-    rEy[s3][ty][tx + 2] = cvt(S.ey, iey);
-    rEz[s3][ty + 2][tx] = cvt(S.ez, iez);
+    // publish plane i (Ex) and plane i+1 (Ey, Ez); H operands to the own slots
+    float v[V];
+    unpackv(Q.ex, iex, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sEx[b][ty + 2][c0 + q + 2] = v[q];
+    unpackv(Q.ey, iey, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rEy[s3][ty][c0 + q + 2] = v[q];
+    unpackv(Q.ez, iez, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rEz[s3][ty + 2][c0 + q] = v[q];
     if (yw) {
-      sEx[b][yrow][tx + 2] = cvt(S.exy, iex);
-      rEz[s3][yrow][tx] = cvt(S.ezy, iez);
+      unpackv(Q.exy, iex, v);
+#pragma unroll
+      for (int q = 0; q < V; q++) sEx[b][yrow][c0 + q + 2] = v[q];
+      unpackv(Q.ezy, iez, v);
+#pragma unroll
+      for (int q = 0; q < V; q++) rEz[s3][yrow][c0 + q] = v[q];
     }
     if (zl) {
-      sEx[b][ty + 2][zcol] = cvt(S.exz, iex);
-      rEy[s3][ty][zcol] = cvt(S.eyz, iey);
+      sEx[b][ty + 2][zcol] = cvt(Q.exz, iex);
+      rEy[s3][ty][zcol] = cvt(Q.eyz, iey);
     }
-    // the H operands go through this thread's own LDS slot: no register of
+    // the H operands go through this thread's own LDS slots: no register of
     // the prefetch set is live across its refetch
-    sHo[0][ty][tx] = cvt(S.hx, ihx);
-    sHo[1][ty][tx] = cvt(S.hy, ihy);
-    sHo[2][ty][tx] = cvt(S.hz, ihz);
+    unpackv(Q.hx, ihx, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sHo[0][ty][c0 + q] = v[q];
+    unpackv(Q.hy, ihy, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sHo[1][ty][c0 + q] = v[q];
+    unpackv(Q.hz, ihz, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sHo[2][ty][c0 + q] = v[q];
     // CPML psi of plane i, loaded before the prefetch so that waiting for
-    // psi does not wait for the prefetch (loads complete in issue order)
-    float pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f, py0 = 0.f, py1 = 0.f;
-    long long qz = 0, qx = 0, qy = 0;
+    // psi does not wait for it (loads complete in issue order)
+    // (z and y slabs; the x slab is a few planes, its psi is read in place)
+    float pz0[V], pz1[V], py0[V], py1[V];
+    const unsigned zb = psi_base_z(g, pm, i, j), yb = psi_base_y(g, pm, ysl, i, k);
+#pragma unroll
+    for (int q = 0; q < V; q++) {
+      pz0[q] = pz1[q] = py0[q] = py1[q] = 0.f;
+      if (!SP && inreg && zs[q] >= 0) {
+        pz0[q] = pm.psiH[2][0][zb + zs[q]];
+        pz1[q] = pm.psiH[2][1][zb + zs[q]];
+      }
+      if (!SP && GY && inreg && ysl >= 0) {
+        py0[q] = pm.psiH[1][0][yb + q];
+        py1[q] = pm.psiH[1][1][yb + q];
+      }
+    }
     const int xsl = (GX && pm.on) ? slab_index(i, g.nx, pm.plo[0], pm.phi[0]) : -1;
-    if (inreg && zs >= 0) {
-      qz = psi_index(g, pm, 2, zs, i, j, k);
-      pz0 = pm.psiH[2][0][qz];
-      pz1 = pm.psiH[2][1][qz];
-    }
-    float bx = 0.f, cx = 0.f;
-    if (GX && xsl >= 0) {
-      bx = pm.bH[0][i];
-      cx = pm.cH[0][i];
-    }
-    if (GX && inreg && xsl >= 0) {
-      qx = psi_index(g, pm, 0, xsl, i, j, k);
-      px0 = pm.psiH[0][0][qx];
-      px1 = pm.psiH[0][1][qx];
-    }
-    if (GY && inreg && ysl >= 0) {
-      qy = psi_index(g, pm, 1, ysl, i, j, k);
-      py0 = pm.psiH[1][0][qy];
-      py1 = pm.psiH[1][1][qy];
-    }
-    fetch(S, min(i + 2, t.i1 - 1));
+    fetch(Q, min(i + PF, t.i1 - 1));
     lds_barrier();
     if (inreg) {
-      const float hx0 = sHo[0][ty][tx], hy0 = sHo[1][ty][tx], hz0 = sHo[2][ty][tx];
-      const float exo = sEx[b][ty + 2][tx + 2];
-      float dyEz = rd1 * d24(rEz[s2][ty][tx], rEz[s2][ty + 1][tx], rEz[s2][ty + 2][tx], rEz[s2][ty + 3][tx]);
-      float dyEx = rd1 * d24(sEx[b][ty][tx + 2], sEx[b][ty + 1][tx + 2], exo, sEx[b][ty + 3][tx + 2]);
-      float dzEy = rd2 * d24(rEy[s2][ty][tx], rEy[s2][ty][tx + 1], rEy[s2][ty][tx + 2], rEy[s2][ty][tx + 3]);
-      float dzEx = rd2 * d24(sEx[b][ty + 2][tx], sEx[b][ty + 2][tx + 1], exo, sEx[b][ty + 2][tx + 3]);
-      float dxEz = rd0 * d24(rEz[s0][ty + 2][tx], rEz[s1][ty + 2][tx], rEz[s2][ty + 2][tx], rEz[s3][ty + 2][tx]);
-      float dxEy = rd0 * d24(rEy[s0][ty][tx + 2], rEy[s1][ty][tx + 2], rEy[s2][ty][tx + 2], rEy[s3][ty][tx + 2]);
-      if (zs >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
-        cpml_step(pm.psiH[2][0], qz, pz0, bz, cz, dzEy);
-        cpml_step(pm.psiH[2][1], qz, pz1, bz, cz, dzEx);
+      float nx_[V], ny_[V], nz_[V];
+#pragma unroll
+      for (int q = 0; q < V; q++) {
+        const int c = c0 + q;
+        const float hx0 = sHo[0][ty][c], hy0 = sHo[1][ty][c], hz0 = sHo[2][ty][c];
+        const float exo = sEx[b][ty + 2][c + 2];
+        float dyEz = rd1 * d24(rEz[s2][ty][c], rEz[s2][ty + 1][c], rEz[s2][ty + 2][c], rEz[s2][ty + 3][c]);
+        float dyEx = rd1 * d24(sEx[b][ty][c + 2], sEx[b][ty + 1][c + 2], exo, sEx[b][ty + 3][c + 2]);
+        float dzEy = rd2 * d24(rEy[s2][ty][c], rEy[s2][ty][c + 1], rEy[s2][ty][c + 2], rEy[s2][ty][c + 3]);
+        float dzEx = rd2 * d24(sEx[b][ty + 2][c], sEx[b][ty + 2][c + 1], exo, sEx[b][ty + 2][c + 3]);
+        float dxEz = rd0 * d24(rEz[s0][ty + 2][c], rEz[s1][ty + 2][c], rEz[s2][ty + 2][c], rEz[s3][ty + 2][c]);
+        float dxEy = rd0 * d24(rEy[s0][ty][c + 2], rEy[s1][ty][c + 2], rEy[s2][ty][c + 2], rEy[s3][ty][c + 2]);
+        if constexpr (SP) {
+          // split parts, (k1, k2) = (1, 1) off the slab: Hx (y, z), Hy (z, x), Hz (x, y)
+          const bool sx = GX && xsl >= 0;
+          const float k1x = sx ? pm.bH[0][i] : 1.f, k2x = sx ? pm.cH[0][i] : 1.f;
+          const float k1y = (GY && ysl >= 0) ? by : 1.f, k2y = (GY && ysl >= 0) ? cy : 1.f;
+          const float k1z = zs[q] >= 0 ? bz[q] : 1.f, k2z = zs[q] >= 0 ? cz[q] : 1.f;
+          const long long qs = plane_base<GX>(g, i) + own + q;
+          nx_[q] = split_step(pm.p2H[0], qs, hx0, k1y, k2y, -dt * dyEz, k1z, k2z, dt * dzEy);
+          ny_[q] = split_step(pm.p2H[1], qs, hy0, k1z, k2z, -dt * dzEx, k1x, k2x, dt * dxEz);
+          nz_[q] = split_step(pm.p2H[2], qs, hz0, k1x, k2x, -dt * dxEy, k1y, k2y, dt * dyEx);
+        } else {
+          if (zs[q] >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
+            cpml_step(pm.psiH[2][0], zb + zs[q], pz0[q], bz[q], cz[q], dzEy);
+            cpml_step(pm.psiH[2][1], zb + zs[q], pz1[q], bz[q], cz[q], dzEx);
+          }
+          if (GX && xsl >= 0) {  // x axis: slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
+            const unsigned qx = psi_base_x(g, pm, xsl, j, k) + q;
+            cpml_apply(pm.psiH[0][0], qx, pm.bH[0][i], pm.cH[0][i], dxEz);
+            cpml_apply(pm.psiH[0][1], qx, pm.bH[0][i], pm.cH[0][i], dxEy);
+          }
+          if (GY && ysl >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
+            cpml_step(pm.psiH[1][0], yb + q, py0[q], by, cy, dyEx);
+            cpml_step(pm.psiH[1][1], yb + q, py1[q], by, cy, dyEz);
+          }
+          nx_[q] = hx0 - dt * (dyEz - dzEy);
+          ny_[q] = hy0 - dt * (dzEx - dxEz);
+          nz_[q] = hz0 - dt * (dxEy - dyEx);
+        }
       }
-      if (GX && xsl >= 0) {  // x axis: slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
-        cpml_step(pm.psiH[0][0], qx, px0, bx, cx, dxEz);
-        cpml_step(pm.psiH[0][1], qx, px1, bx, cx, dxEy);
+      const long long qq = plane_base<GX>(g, i) + own;
+      if constexpr (V == 2) {
+        stv2(Hx, qq, nx_, sc.sh[0]);
+        stv2(Hy, qq, ny_, sc.sh[1]);
+        stv2(Hz, qq, nz_, sc.sh[2]);
+      } else {
+        stv(Hx, qq, nx_, sc.sh[0]);
+        stv(Hy, qq, ny_, sc.sh[1]);
+        stv(Hz, qq, nz_, sc.sh[2]);
       }
-      if (GY && ysl >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
-        cpml_step(pm.psiH[1][0], qy, py0, by, cy, dyEx);
-        cpml_step(pm.psiH[1][1], qy, py1, by, cy, dyEz);
-      }
-      const long long q = plane_base<GX>(g, i) + own;
-      st(Hx, q, hx0 - dt * (dyEz - dzEy), sc.sh[0]);
-      st(Hy, q, hy0 - dt * (dzEx - dxEz), sc.sh[1]);
-      st(Hz, q, hz0 - dt * (dxEy - dyEx), sc.sh[2]);
     }
   };
   int i = t.i0;
-  for (; i + 1 < t.i1; i += 2) {
-    step(i, A);
-    step(i + 1, B);
+  for (; i + PF - 1 < t.i1; i += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) step(i + u, S[u]);
   }
-  if (i < t.i1) step(i, A);
+#pragma unroll
+  for (int u = 0; u < PF - 1; u++)
+    if (i + u < t.i1) step(i + u, S[u]);
 }
 
 // ============================================================ ADE cells
@@ -583,6 +713,7 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
   const int vpos[3] = {i, j, k};
   static constexpr int off[6][3] = {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   float dc[3] = {0.f, 0.f, 0.f};  // curl correction per component, all axes
+  float dc2[3] = {0.f, 0.f, 0.f}; // its part on the stored split-field part (second curl axis)
   for (int rb = r; rb < bands.n; rb++) {
     if (rb != r && !holds(bands.r[rb])) continue;
     const int a = bands.axis[rb];
@@ -597,7 +728,15 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
       const float D = g.rd[a] * tfsf_dcorr<TAB>(tf, hx, hy, hz, a, tinc);
       if (D == 0.0f) continue;
       float cp = 0.0f;
-      if (pm.on) {
+      if (pm.on && pm.split) {  // split-field layer: the part of axis a gets k2 * D
+        const int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+        if (slab_index(vpos[a], n, pm.plo[a], pm.phi[a]) >= 0) {
+          const float k2 = HALF == 0 ? pm.cH[a][vpos[a]] : pm.cE[a][vpos[a]];
+          dc[c] += sgn * k2 * D;
+          if (t == 1) dc2[c] += sgn * k2 * D;
+          continue;
+        }
+      } else if (pm.on) {
         const int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
         const int si = slab_index(vpos[a], n, pm.plo[a], pm.phi[a]);
         if (si >= 0) {
@@ -616,8 +755,10 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
     }
   }
   if (HALF == 0) {
-    for (int c = 0; c < 3; c++)
+    for (int c = 0; c < 3; c++) {
       if (dc[c] != 0.0f) st(f.H[c], q, ld(f.H[c], q, sc.ih[c]) - g.dt * dc[c], sc.sh[c]);
+      if (dc2[c] != 0.0f) pm.p2H[c][q] -= g.dt * dc2[c];
+    }
     return;
   }
   const ClassEntry ce = mt.table[mt.cls ? mt.cls[q] : 0];
@@ -641,6 +782,7 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
     } else {
       const float ie = over ? dv : ce.inv_eps[c];
       st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + g.dt * ie * dc[c], sc.se[c]);
+      if (dc2[c] != 0.0f) pm.p2E[c][q] += g.dt * ie * dc2[c];
     }
   }
 }
@@ -666,40 +808,49 @@ __global__ __launch_bounds__(256) void k_tfsf_table(Tfsf tf, double* out, long l
 }
 
 // ================================================================ E sweep
-// Operands of one plane, loaded two planes ahead: the Hx tile values of
-// plane p, the E read-modify-write operands and class byte of plane p, and
-// the Hy / Hz tile values of plane p+2 (the x stencil of E reaches two
-// planes ahead).
-template <typename T> struct PreE {
-  T hx, hxy, hxz, hy, hyz, hz, hzy, ex, ey, ez;
-  int cls;
+// Operands of one plane, loaded PF planes ahead: the Hx tile words of
+// plane p (own, y-halo row) and the Hx z-halo value, the E read-modify-write
+// words and class bytes of plane p, and the Hy / Hz tile values of plane p+2
+// (the x stencil of E reaches two planes ahead).
+template <typename P, typename T, typename CL> struct PreE {
+  P hx, hxy, hy, hz, hzy, ex, ey, ez;
+  T hxz, hyz;
+  CL cls;
 };
+template <int V> struct ClsWord;
+template <> struct ClsWord<1> { using T = uint8_t; };
+template <> struct ClsWord<2> { using T = unsigned short; };
 
-template <int STORE, int GEN>
-__global__ __launch_bounds__(BZ* MBY) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
-                                                      Ade ade, MarchSet ms) {
+template <int STORE, int GEN, int V, bool SP = false>
+__global__ __launch_bounds__(BZ* MBY, 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
+                                                         Ade ade, MarchSet ms) {
   using T = typename Raw<STORE>::T;
+  using P = typename Vec<STORE, V>::P;
+  using CL = typename ClsWord<V>::T;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
+  constexpr bool DES = V == 1;  // design-box support (its launches use one cell per lane)
+  constexpr int BZV = BZ * V, LZ = BZV + 3;
   __shared__ ClassEntry tbl[kMaxClasses];
-  __shared__ float sHx[2][MBY + 3][MLZ];  // [row j-j0+1][col k-k0+1], plane i
-  __shared__ float rHy[5][MBY][MLZ];      // [row j-j0][col k-k0+1], ring of planes
-  __shared__ float rHz[5][MBY + 3][BZ];   // [row j-j0+1][col k-k0], ring of planes
-  __shared__ float sEo[3][MBY][BZ];       // this thread's E operands of plane i
-  __shared__ int sCl[MBY][BZ];            // and its class byte
+  __shared__ float sHx[2][MBY + 3][LZ];  // [row j-j0+1][col k-k0+1], plane i
+  __shared__ float rHy[5][MBY][LZ];      // [row j-j0][col k-k0+1], ring of planes
+  __shared__ float rHz[5][MBY + 3][BZV]; // [row j-j0+1][col k-k0], ring of planes
+  __shared__ float sEo[3][MBY][BZV];     // this thread's E operands of plane i
+  __shared__ uint8_t sCl[MBY][BZV];      // and its class bytes
   const int tx = threadIdx.x, ty = threadIdx.y;
   for (int q = tx + ty * BZ; q < mt.nclass; q += BZ * MBY) tbl[q] = mt.table[q];
   MarchTile t;
   if (!march_locate(ms, t)) return;  // block-uniform
   const Region& R = ms.r[t.r];
-  const int j = t.j0 + ty, k = t.k0 + tx;
+  const int j = t.j0 + ty, k = t.k0 + V * tx;
+  const int c0 = V * tx;
   const bool inreg = j < R.j1 && k < R.k1;
   const long long own = plane_off(g, j, k);
-  // halo rows j0-1, j0+MBY, j0+MBY+1 (waves ty<3); columns k0-1, k0+64, k0+65 (lanes tx<3)
+  // halo rows j0-1, j0+MBY, j0+MBY+1 (waves ty<3); columns k0-1, k0+BZV, k0+BZV+1 (lanes tx<3)
   const bool yw = ty < 3, zl = tx < 3;
   const int yrow = ty == 0 ? 0 : MBY + ty;
   const long long yh = yw ? plane_off(g, t.j0 + (ty == 0 ? -1 : MBY + ty - 1), k) : -1;
-  const int zcol = tx == 0 ? 0 : BZ + tx;
-  const long long zh = zl ? plane_off(g, j, t.k0 + (tx == 0 ? -1 : BZ + tx - 1)) : -1;
+  const int zcol = tx == 0 ? 0 : BZV + tx;
+  const long long zh = zl ? plane_off(g, j, t.k0 + (tx == 0 ? -1 : BZV + tx - 1)) : -1;
   const T* Hx = f.H[0];
   const T* Hy = f.H[1];
   const T* Hz = f.H[2];
@@ -708,141 +859,225 @@ __global__ __launch_bounds__(BZ* MBY) void k_update_e(Grid g, Fields<STORE> f, S
   T* Ez = f.E[2];
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
-  const T z0 = rzero<T>();
 
   const long long eq = inreg ? own : -1;  // E operands: owned cells only
+  // with no class array every cell is class 0: read the zero page
   const uint8_t* clsp = mt.cls ? mt.cls : static_cast<const uint8_t*>(g.zero);
-  const long long cq = mt.cls ? eq : -1;
-  auto fetch = [&](PreE<T>& S, int p) {
+  const long long cq = mt.cls ? eq : 0;
+  const unsigned czidx = mt.cls ? g.zidx : 0u;
+  auto fetch = [&](PreE<P, T, CL>& S, int p) {
     const long long pb = plane_base<GX>(g, p);
     const long long pn = plane_base<GX>(g, p + 2);
-    S.hx = *at(Hx, pb, own, g.zero);
-    S.hy = *at(Hy, pn, own, g.zero);
-    S.hz = *at(Hz, pn, own, g.zero);
-    S.hxy = *at(Hx, pb, yh, g.zero);
-    S.hzy = *at(Hz, pn, yh, g.zero);
-    S.hxz = *at(Hx, pb, zh, g.zero);
-    S.hyz = *at(Hy, pn, zh, g.zero);
-    S.ex = *at((const T*)Ex, pb, eq, g.zero);
-    S.ey = *at((const T*)Ey, pb, eq, g.zero);
-    S.ez = *at((const T*)Ez, pb, eq, g.zero);
-    S.cls = *at(clsp, pb, cq, g.zero);
+    S.hx = ldv<P>(Hx, pb, own, g.zidx);
+    S.hy = ldv<P>(Hy, pn, own, g.zidx);
+    S.hz = ldv<P>(Hz, pn, own, g.zidx);
+    S.hxy = ldv<P>(Hx, pb, yh, g.zidx);
+    S.hzy = ldv<P>(Hz, pn, yh, g.zidx);
+    S.hxz = *at(Hx, pb, zh, g.zidx);
+    S.hyz = *at(Hy, pn, zh, g.zidx);
+    S.ex = ldv<P>((const T*)Ex, pb, eq, g.zidx);
+    S.ey = ldv<P>((const T*)Ey, pb, eq, g.zidx);
+    S.ez = ldv<P>((const T*)Ez, pb, eq, g.zidx);
+    S.cls = ldv<CL>(clsp, mt.cls ? pb : 0, cq, czidx);
   };
   auto ring_direct = [&](int p) {  // Hy / Hz tile of plane p into its slot
     const long long pb = plane_base<GX>(g, p);
     const int s = ring5(p);
-    rHy[s][ty][tx + 1] = cvt(ldr(Hy, pb, own), ihy);
-    rHz[s][ty + 1][tx] = cvt(ldr(Hz, pb, own), ihz);
-    if (yw) rHz[s][yrow][tx] = cvt(ldr(Hz, pb, yh), ihz);
+    float v[V];
+    unpackv(ldv<P>(Hy, pb, own, g.zidx), ihy, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rHy[s][ty][c0 + q + 1] = v[q];
+    unpackv(ldv<P>(Hz, pb, own, g.zidx), ihz, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rHz[s][ty + 1][c0 + q] = v[q];
+    if (yw) {
+      unpackv(ldv<P>(Hz, pb, yh, g.zidx), ihz, v);
+#pragma unroll
+      for (int q = 0; q < V; q++) rHz[s][yrow][c0 + q] = v[q];
+    }
     if (zl) rHy[s][ty][zcol] = cvt(ldr(Hy, pb, zh), ihy);
   };
 
-  const int zs = pm.on ? slab_index(k, g.nz, pm.plo[2], pm.phi[2]) : -1;
   const int ysl = (GY && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
-  const bool design_col = mt.d_on && j >= mt.d_j0 && j < mt.d_j0 + mt.d_nj && k >= mt.d_k0 && k < mt.d_k0 + mt.d_nk;
+  bool design_col[V];
+  int zs[V];
+  float bz[V], cz[V];
+#pragma unroll
+  for (int q = 0; q < V; q++) {
+    design_col[q] = DES && mt.d_on && j >= mt.d_j0 && j < mt.d_j0 + mt.d_nj && k + q >= mt.d_k0 &&
+                    k + q < mt.d_k0 + mt.d_nk;
+    zs[q] = pm.on ? slab_index(k + q, g.nz, pm.plo[2], pm.phi[2]) : -1;
+    bz[q] = zs[q] >= 0 ? pm.bE[2][k + q] : 0.f;
+    cz[q] = zs[q] >= 0 ? pm.cE[2][k + q] : 0.f;
+  }
   const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
-  const float bz = zs >= 0 ? pm.bE[2][k] : 0.f, cz = zs >= 0 ? pm.cE[2][k] : 0.f;
   const float by = ysl >= 0 ? pm.bE[1][j] : 0.f, cy = ysl >= 0 ? pm.cE[1][j] : 0.f;
 
-  PreE<T> A{z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, 0}, B = A;
+  constexpr int PF = (GY || V == 2) ? 2 : 3;  // prefetch depth, as in the H sweep
+  PreE<P, T, CL> S[PF];
+#pragma unroll
+  for (int u = 0; u < PF; u++) {
+    S[u].hx = S[u].hxy = S[u].hy = S[u].hz = S[u].hzy = S[u].ex = S[u].ey = S[u].ez = pzero<P>();
+    S[u].hxz = S[u].hyz = rzero<T>();
+    S[u].cls = 0;
+  }
   ring_direct(t.i0 - 1);
   ring_direct(t.i0);
   ring_direct(t.i0 + 1);
-  fetch(A, t.i0);  // unconditional prefetches, as in the H sweep
-  fetch(B, min(t.i0 + 1, t.i1 - 1));
+#pragma unroll
+  for (int u = 0; u < PF; u++) fetch(S[u], min(t.i0 + u, t.i1 - 1));  // unconditional, as in the H sweep
 
-  auto step = [&](int i, PreE<T>& S) {
+  auto step = [&](int i, PreE<P, T, CL>& Q) {
     const int b = i & 1;
     const int s0 = ring5(i - 1), s1 = ring5(i), s2 = ring5(i + 1), s3 = ring5(i + 2);
-    // publish plane i (Hx) and plane i+2 (Hy, Hz); E operands to the own slot
-    sHx[b][ty + 1][tx + 1] = cvt(S.hx, ihx);
-    rHy[s3][ty][tx + 1] = cvt(S.hy, ihy);
-    rHz[s3][ty + 1][tx] = cvt(S.hz, ihz);
+    // publish plane i (Hx) and plane i+2 (Hy, Hz); E operands to the own slots
+    float v[V];
+    unpackv(Q.hx, ihx, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sHx[b][ty + 1][c0 + q + 1] = v[q];
+    unpackv(Q.hy, ihy, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rHy[s3][ty][c0 + q + 1] = v[q];
+    unpackv(Q.hz, ihz, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) rHz[s3][ty + 1][c0 + q] = v[q];
     if (yw) {
-      sHx[b][yrow][tx + 1] = cvt(S.hxy, ihx);
-      rHz[s3][yrow][tx] = cvt(S.hzy, ihz);
+      unpackv(Q.hxy, ihx, v);
+#pragma unroll
+      for (int q = 0; q < V; q++) sHx[b][yrow][c0 + q + 1] = v[q];
+      unpackv(Q.hzy, ihz, v);
+#pragma unroll
+      for (int q = 0; q < V; q++) rHz[s3][yrow][c0 + q] = v[q];
     }
     if (zl) {
-      sHx[b][ty + 1][zcol] = cvt(S.hxz, ihx);
-      rHy[s3][ty][zcol] = cvt(S.hyz, ihy);
+      sHx[b][ty + 1][zcol] = cvt(Q.hxz, ihx);
+      rHy[s3][ty][zcol] = cvt(Q.hyz, ihy);
     }
-    sEo[0][ty][tx] = cvt(S.ex, iex);
-    sEo[1][ty][tx] = cvt(S.ey, iey);
-    sEo[2][ty][tx] = cvt(S.ez, iez);
-    sCl[ty][tx] = S.cls;
+    unpackv(Q.ex, iex, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sEo[0][ty][c0 + q] = v[q];
+    unpackv(Q.ey, iey, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sEo[1][ty][c0 + q] = v[q];
+    unpackv(Q.ez, iez, v);
+#pragma unroll
+    for (int q = 0; q < V; q++) sEo[2][ty][c0 + q] = v[q];
+#pragma unroll
+    for (int q = 0; q < V; q++) sCl[ty][c0 + q] = (uint8_t)(Q.cls >> (8 * q));
     // psi and design-box operands of plane i before the prefetch
-    float pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f, py0 = 0.f, py1 = 0.f, dv = 0.f;
-    long long qz = 0, qx = 0, qy = 0;
+    float pz0[V], pz1[V], py0[V], py1[V], dv[V];
+    bool over[V];
+    const unsigned zb = psi_base_z(g, pm, i, j), yb = psi_base_y(g, pm, ysl, i, k);
+#pragma unroll
+    for (int q = 0; q < V; q++) {
+      pz0[q] = pz1[q] = py0[q] = py1[q] = dv[q] = 0.f;
+      if (!SP && inreg && zs[q] >= 0) {
+        pz0[q] = pm.psiE[2][0][zb + zs[q]];
+        pz1[q] = pm.psiE[2][1][zb + zs[q]];
+      }
+      if (!SP && GY && inreg && ysl >= 0) {
+        py0[q] = pm.psiE[1][0][yb + q];
+        py1[q] = pm.psiE[1][1][yb + q];
+      }
+      over[q] = DES && inreg && design_col[q] && i >= mt.d_i0 && i < mt.d_i0 + mt.d_ni;
+      if (DES && over[q])
+        dv[q] = mt.d_inv_eps[((long long)(i - mt.d_i0) * mt.d_nj + (j - mt.d_j0)) * mt.d_nk + (k + q - mt.d_k0)];
+    }
     const int xsl = (GX && pm.on) ? slab_index(i, g.nx, pm.plo[0], pm.phi[0]) : -1;
-    if (inreg && zs >= 0) {
-      qz = psi_index(g, pm, 2, zs, i, j, k);
-      pz0 = pm.psiE[2][0][qz];
-      pz1 = pm.psiE[2][1][qz];
-    }
-    float bx = 0.f, cx = 0.f;
-    if (GX && xsl >= 0) {
-      bx = pm.bE[0][i];
-      cx = pm.cE[0][i];
-    }
-    if (GX && inreg && xsl >= 0) {
-      qx = psi_index(g, pm, 0, xsl, i, j, k);
-      px0 = pm.psiE[0][0][qx];
-      px1 = pm.psiE[0][1][qx];
-    }
-    if (GY && inreg && ysl >= 0) {
-      qy = psi_index(g, pm, 1, ysl, i, j, k);
-      py0 = pm.psiE[1][0][qy];
-      py1 = pm.psiE[1][1][qy];
-    }
-    const bool over = inreg && design_col && i >= mt.d_i0 && i < mt.d_i0 + mt.d_ni;
-    if (over) dv = mt.d_inv_eps[((long long)(i - mt.d_i0) * mt.d_nj + (j - mt.d_j0)) * mt.d_nk + (k - mt.d_k0)];
-    fetch(S, min(i + 2, t.i1 - 1));
+    fetch(Q, min(i + PF, t.i1 - 1));
     lds_barrier();
     if (inreg) {
-      const float ex0 = sEo[0][ty][tx], ey0 = sEo[1][ty][tx], ez0 = sEo[2][ty][tx];
-      const int cls = sCl[ty][tx];
-      const float hxo = sHx[b][ty + 1][tx + 1];
-      float dyHz = rd1 * d24(rHz[s1][ty][tx], rHz[s1][ty + 1][tx], rHz[s1][ty + 2][tx], rHz[s1][ty + 3][tx]);
-      float dyHx = rd1 * d24(sHx[b][ty][tx + 1], hxo, sHx[b][ty + 2][tx + 1], sHx[b][ty + 3][tx + 1]);
-      float dzHy = rd2 * d24(rHy[s1][ty][tx], rHy[s1][ty][tx + 1], rHy[s1][ty][tx + 2], rHy[s1][ty][tx + 3]);
-      float dzHx = rd2 * d24(sHx[b][ty + 1][tx], hxo, sHx[b][ty + 1][tx + 2], sHx[b][ty + 1][tx + 3]);
-      float dxHz = rd0 * d24(rHz[s0][ty + 1][tx], rHz[s1][ty + 1][tx], rHz[s2][ty + 1][tx], rHz[s3][ty + 1][tx]);
-      float dxHy = rd0 * d24(rHy[s0][ty][tx + 1], rHy[s1][ty][tx + 1], rHy[s2][ty][tx + 1], rHy[s3][ty][tx + 1]);
-      if (zs >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
-        cpml_step(pm.psiE[2][0], qz, pz0, bz, cz, dzHy);
-        cpml_step(pm.psiE[2][1], qz, pz1, bz, cz, dzHx);
+      float nE[3][V];
+      bool plain = true;
+#pragma unroll
+      for (int q = 0; q < V; q++) {
+        const int c = c0 + q;
+        const float ex0 = sEo[0][ty][c], ey0 = sEo[1][ty][c], ez0 = sEo[2][ty][c];
+        const int cls = sCl[ty][c];
+        const float hxo = sHx[b][ty + 1][c + 1];
+        float dyHz = rd1 * d24(rHz[s1][ty][c], rHz[s1][ty + 1][c], rHz[s1][ty + 2][c], rHz[s1][ty + 3][c]);
+        float dyHx = rd1 * d24(sHx[b][ty][c + 1], hxo, sHx[b][ty + 2][c + 1], sHx[b][ty + 3][c + 1]);
+        float dzHy = rd2 * d24(rHy[s1][ty][c], rHy[s1][ty][c + 1], rHy[s1][ty][c + 2], rHy[s1][ty][c + 3]);
+        float dzHx = rd2 * d24(sHx[b][ty + 1][c], hxo, sHx[b][ty + 1][c + 2], sHx[b][ty + 1][c + 3]);
+        float dxHz = rd0 * d24(rHz[s0][ty + 1][c], rHz[s1][ty + 1][c], rHz[s2][ty + 1][c], rHz[s3][ty + 1][c]);
+        float dxHy = rd0 * d24(rHy[s0][ty][c + 1], rHy[s1][ty][c + 1], rHy[s2][ty][c + 1], rHy[s3][ty][c + 1]);
+        if (!SP && zs[q] >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
+          cpml_step(pm.psiE[2][0], zb + zs[q], pz0[q], bz[q], cz[q], dzHy);
+          cpml_step(pm.psiE[2][1], zb + zs[q], pz1[q], bz[q], cz[q], dzHx);
+        }
+        if (!SP && GX && xsl >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
+          const unsigned qx = psi_base_x(g, pm, xsl, j, k) + q;
+          cpml_apply(pm.psiE[0][0], qx, pm.bE[0][i], pm.cE[0][i], dxHz);
+          cpml_apply(pm.psiE[0][1], qx, pm.bE[0][i], pm.cE[0][i], dxHy);
+        }
+        if (!SP && GY && ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
+          cpml_step(pm.psiE[1][0], yb + q, py0[q], by, cy, dyHx);
+          cpml_step(pm.psiE[1][1], yb + q, py1[q], by, cy, dyHz);
+        }
+        const float crx = dyHz - dzHy, cry = dzHx - dxHz, crz = dxHy - dyHx;
+        const ClassEntry& ce = tbl[cls];
+        float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
+        if (over[q]) ie0 = ie1 = ie2 = dv[q];
+        if (ce.special && !over[q]) {  // finished by k_ade; its E word keeps the old value here
+          const long long bq =
+              (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k + q - ade.k0);
+          ade.curl[0][bq] = crx;
+          ade.curl[1][bq] = cry;
+          ade.curl[2][bq] = crz;
+          nE[0][q] = ex0;
+          nE[1][q] = ey0;
+          nE[2][q] = ez0;
+          plain = false;
+        } else if constexpr (SP) {
+          // split parts: Ex (y, z), Ey (z, x), Ez (x, y); dispersive cells never sit in the layer
+          const bool sx = GX && xsl >= 0;
+          const float k1x = sx ? pm.bE[0][i] : 1.f, k2x = sx ? pm.cE[0][i] : 1.f;
+          const float k1y = (GY && ysl >= 0) ? by : 1.f, k2y = (GY && ysl >= 0) ? cy : 1.f;
+          const float k1z = zs[q] >= 0 ? bz[q] : 1.f, k2z = zs[q] >= 0 ? cz[q] : 1.f;
+          const long long qs = plane_base<GX>(g, i) + own + q;
+          nE[0][q] = split_step(pm.p2E[0], qs, ex0, k1y, k2y, dt * ie0 * dyHz, k1z, k2z, -dt * ie0 * dzHy);
+          nE[1][q] = split_step(pm.p2E[1], qs, ey0, k1z, k2z, dt * ie1 * dzHx, k1x, k2x, -dt * ie1 * dxHz);
+          nE[2][q] = split_step(pm.p2E[2], qs, ez0, k1x, k2x, dt * ie2 * dxHy, k1y, k2y, -dt * ie2 * dyHx);
+        } else {
+          nE[0][q] = ex0 + dt * ie0 * crx;
+          nE[1][q] = ey0 + dt * ie1 * cry;
+          nE[2][q] = ez0 + dt * ie2 * crz;
+        }
       }
-      if (GX && xsl >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
-        cpml_step(pm.psiE[0][0], qx, px0, bx, cx, dxHz);
-        cpml_step(pm.psiE[0][1], qx, px1, bx, cx, dxHy);
-      }
-      if (GY && ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
-        cpml_step(pm.psiE[1][0], qy, py0, by, cy, dyHx);
-        cpml_step(pm.psiE[1][1], qy, py1, by, cy, dyHz);
-      }
-      const float crx = dyHz - dzHy, cry = dzHx - dxHz, crz = dxHy - dyHx;
-      const long long q = plane_base<GX>(g, i) + own;
-      const ClassEntry& ce = tbl[cls];
-      float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
-      if (over) ie0 = ie1 = ie2 = dv;
-      if (ce.special && !over) {  // finished by k_ade
-        const long long bq = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
-        ade.curl[0][bq] = crx;
-        ade.curl[1][bq] = cry;
-        ade.curl[2][bq] = crz;
+      const long long qq = plane_base<GX>(g, i) + own;
+      if constexpr (V == 2) {
+        // a pair with a dispersive cell stores only its plain cells: k_ade
+        // writes the other one (from the old value it reads)
+        if (plain) {
+          stv2(Ex, qq, nE[0], sc.se[0]);
+          stv2(Ey, qq, nE[1], sc.se[1]);
+          stv2(Ez, qq, nE[2], sc.se[2]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < V; q++) {
+            if (tbl[sCl[ty][c0 + q]].special && !over[q]) continue;
+            stv(Ex + q, qq, &nE[0][q], sc.se[0]);
+            stv(Ey + q, qq, &nE[1][q], sc.se[1]);
+            stv(Ez + q, qq, &nE[2][q], sc.se[2]);
+          }
+        }
       } else {
-        st(Ex, q, ex0 + dt * ie0 * crx, sc.se[0]);
-        st(Ey, q, ey0 + dt * ie1 * cry, sc.se[1]);
-        st(Ez, q, ez0 + dt * ie2 * crz, sc.se[2]);
+        if (plain) {
+          stv(Ex, qq, nE[0], sc.se[0]);
+          stv(Ey, qq, nE[1], sc.se[1]);
+          stv(Ez, qq, nE[2], sc.se[2]);
+        }
       }
     }
   };
   int i = t.i0;
-  for (; i + 1 < t.i1; i += 2) {
-    step(i, A);
-    step(i + 1, B);
+  for (; i + PF - 1 < t.i1; i += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) step(i + u, S[u]);
   }
-  if (i < t.i1) step(i, A);
+#pragma unroll
+  for (int u = 0; u < PF - 1; u++)
+    if (i + u < t.i1) step(i + u, S[u]);
 }
 
 // ======================================================== small kernels
@@ -959,14 +1194,14 @@ static int resident_blocks(K kernel) {
 // about kWarm planes (three ring planes loaded synchronously, then the
 // two-plane prefetch filling), and the launch runs in ceil(blocks / slots)
 // rounds.  Minimise rounds * (seg + kWarm).
-static int march_seg(const Region* regs, int nreg, int slots) {
+static int march_seg(const Region* regs, int nreg, int slots, int bz) {
   constexpr int kWarm = 6;
   long long cols = 0;
   int maxlen = 1;
   for (int t = 0; t < nreg; t++) {
     const Region& r = regs[t];
     if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
-    cols += (long long)((r.k1 - r.k0 + BZ - 1) / BZ) * ((r.j1 - r.j0 + MBY - 1) / MBY);
+    cols += (long long)((r.k1 - r.k0 + bz - 1) / bz) * ((r.j1 - r.j0 + MBY - 1) / MBY);
     maxlen = std::max(maxlen, r.i1 - r.i0);
   }
   int best = maxlen;
@@ -978,7 +1213,7 @@ static int march_seg(const Region* regs, int nreg, int slots) {
     for (int t = 0; t < nreg; t++) {
       const Region& r = regs[t];
       if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
-      blocks += (long long)((r.k1 - r.k0 + BZ - 1) / BZ) * ((r.j1 - r.j0 + MBY - 1) / MBY) *
+      blocks += (long long)((r.k1 - r.k0 + bz - 1) / bz) * ((r.j1 - r.j0 + MBY - 1) / MBY) *
                 ((r.i1 - r.i0 + seg - 1) / seg);
     }
     const long long rounds = (blocks + slots - 1) / slots;
@@ -993,57 +1228,80 @@ static int march_seg(const Region* regs, int nreg, int slots) {
   return best;
 }
 
-template <int STORE>
+template <int STORE, int V, bool SP>
 static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
                   const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, MBY, 1);
-  static const int slots_m = resident_blocks(k_update_h<STORE, 1>);
-  static const int slots_e = resident_blocks(k_update_h<STORE, 3>);
+  static const int slots_m = resident_blocks(k_update_h<STORE, 1, V, SP>);
+  static const int slots_e = resident_blocks(k_update_h<STORE, 3, V, SP>);
   const int nm = main.i1 > main.i0 ? 1 : 0;
-  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m));
+  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m, BZ * V), BZ * V);
   if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, 1>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, rm);
+    hipLaunchKernelGGL((k_update_h<STORE, 1, V, SP>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, rm);
     check_launch("k_update_h<main>");
   }
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e));
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e, BZ * V), BZ * V);
   if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, 3>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
+    hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
     check_launch("k_update_h<edge>");
   }
 }
 
-template <int STORE>
+template <int STORE, int V, bool SP>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                   const Ade& ade, const Region* regs, int nreg, const Region& main, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, MBY, 1);
-  static const int slots_m = resident_blocks(k_update_e<STORE, 1>);
-  static const int slots_e = resident_blocks(k_update_e<STORE, 3>);
+  // the y-CPML edge keeps one cell per lane (the paired variant would spill)
+  static const int slots_m = resident_blocks(k_update_e<STORE, 1, V, SP>);
+  static const int slots_e = resident_blocks(k_update_e<STORE, 3, 1, SP>);
   const int nm = main.i1 > main.i0 ? 1 : 0;
-  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m));
+  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m, BZ * V), BZ * V);
   if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, 1>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rm);
+    hipLaunchKernelGGL((k_update_e<STORE, 1, V, SP>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rm);
     check_launch("k_update_e<main>");
   }
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e));
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e, BZ), BZ);
   if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, 3>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
+    hipLaunchKernelGGL((k_update_e<STORE, 3, 1, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
     check_launch("k_update_e<edge>");
   }
 }
 
+// FP16 storage on an even nz runs two z cells per lane (__half2 words);
+// a design box (FP32 inverse design) keeps one cell per lane.
+static bool pair_cells(int store, const Grid& g, const Materials* mt) {
+  return store == 1 && g.nz % 2 == 0 && !(mt && mt->d_on);
+}
+
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                      const Region* regs, int nreg, const Region& main, hipStream_t st) {
-  if (store == 0) upd_h<0>(g, fields, sc, pm, regs, nreg, main, st);
-  else upd_h<1>(g, fields, sc, pm, regs, nreg, main, st);
+  if (pm.split) {  // split-field layer: one cell per lane
+    if (store == 0) upd_h<0, 1, true>(g, fields, sc, pm, regs, nreg, main, st);
+    else upd_h<1, 1, true>(g, fields, sc, pm, regs, nreg, main, st);
+  } else if (store == 0) {
+    upd_h<0, 1, false>(g, fields, sc, pm, regs, nreg, main, st);
+  } else if (pair_cells(store, g, nullptr)) {
+    upd_h<1, 2, false>(g, fields, sc, pm, regs, nreg, main, st);
+  } else {
+    upd_h<1, 1, false>(g, fields, sc, pm, regs, nreg, main, st);
+  }
 }
 
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                      const Materials& mt, const Ade& ade, const Region* regs, int nreg, const Region& main,
                      hipStream_t st) {
-  if (store == 0) upd_e<0>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
-  else upd_e<1>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+  if (pm.split) {
+    if (store == 0) upd_e<0, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+    else upd_e<1, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+  } else if (store == 0) {
+    upd_e<0, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+  } else if (pair_cells(store, g, &mt)) {
+    upd_e<1, 2, false>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+  } else {
+    upd_e<1, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+  }
 }
 
 // One launch per face-normal axis: bands of different axes overlap at the

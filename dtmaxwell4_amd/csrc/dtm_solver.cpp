@@ -459,11 +459,20 @@ static int setup_cpml(dtm_solver* s) {
     p.phi[a] = c.npml[2 * a + 1];
     if (p.plo[a] > 0 || p.phi[a] > 0) p.on = 1;
   }
+  p.split = (p.on && c.pml_kind == DTM_PML_SPLIT) ? 1 : 0;
   for (int a = 0; a < 3; a++) {
     std::vector<float> bE(n[a]), cE(n[a]), bH(n[a]), cH(n[a]);
     for (int v = 0; v < n[a]; v++) {
       const double se = pml_sigma(2 * v + 1, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax);
       const double sh = pml_sigma(2 * v, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax);
+      if (p.split) {  // setPMLcoeffs (DTmxw.cu:250-265)
+        const double k2e = 1.0 / (1.0 + 0.5 * c.dt * se), k2h = 1.0 / (1.0 + 0.5 * c.dt * sh);
+        bE[v] = (float)(2.0 * k2e - 1.0);
+        cE[v] = (float)k2e;
+        bH[v] = (float)(2.0 * k2h - 1.0);
+        cH[v] = (float)k2h;
+        continue;
+      }
       const double be = exp(-se * c.dt), bh = exp(-sh * c.dt);
       bE[v] = (float)be;
       cE[v] = (float)(be - 1.0);
@@ -483,10 +492,18 @@ static int setup_cpml(dtm_solver* s) {
     p.cH[a] = t[3];
     const int P = p.plo[a] + p.phi[a];
     size_t cnt = 0;
-    if (P > 0) cnt = (size_t)P * (a == 0 ? (size_t)c.ny * c.nz : a == 1 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny);
+    if (P > 0 && !p.split)
+      cnt = (size_t)P * (a == 0 ? (size_t)c.ny * c.nz : a == 1 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny);
     for (int sl = 0; sl < 2; sl++) {
       if (int rc = sdalloc(s, &p.psiE[a][sl], sizeof(float) * cnt)) return rc;
       if (int rc = sdalloc(s, &p.psiH[a][sl], sizeof(float) * cnt)) return rc;
+    }
+  }
+  if (p.split) {
+    const size_t cells = (size_t)c.nx * c.ny * c.nz + kPad;
+    for (int q = 0; q < 3; q++) {
+      if (int rc = sdalloc(s, &p.p2E[q], sizeof(float) * cells)) return rc;
+      if (int rc = sdalloc(s, &p.p2H[q], sizeof(float) * cells)) return rc;
     }
   }
   s->pmn = p;
@@ -608,6 +625,12 @@ static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
       any_special = true;
       if (s->classes[id].special & 8) any_gyro = true;
       const int i = (int)(q / ((size_t)c.ny * c.nz)), j = (int)((q / c.nz) % c.ny), k = (int)(q % c.nz);
+      if (s->pm.split) {
+        const int v[3] = {i, j, k}, nn[3] = {c.nx, c.ny, c.nz};
+        for (int a = 0; a < 3; a++)
+          if (v[a] < c.npml[2 * a] || v[a] >= nn[a] - c.npml[2 * a + 1])
+            return fail(DTM_ERR_ARG, "the split-field layer does not cover dispersive or gyrotropic cells");
+      }
       bb[0] = std::min(bb[0], i);
       bb[1] = std::max(bb[1], i);
       bb[2] = std::min(bb[2], j);
@@ -622,7 +645,7 @@ static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
   HIPCHK(hipMemcpy(s->table_dev, s->classes.data(), sizeof(ClassEntry) * ncls, hipMemcpyHostToDevice));
   if (ncls > 1) {
     if (!s->cls_dev)
-      if (int rc = dalloc(s, &s->cls_dev, n)) return rc;
+      if (int rc = dalloc(s, &s->cls_dev, n + kPad)) return rc;
     HIPCHK(hipMemcpy(s->cls_dev, s->cls_host.data(), n, hipMemcpyHostToDevice));
   }
   s->mt.cls = ncls > 1 ? s->cls_dev : nullptr;
@@ -684,6 +707,8 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   *out = nullptr;
   const dtm_config& c = *cfg;
   if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(DTM_ERR_ARG, "grid sizes must be positive");
+  if ((unsigned long long)c.nx * c.ny * c.nz > 0xffffffffULL - kPad)
+    return fail(DTM_ERR_LIMIT, "grids are limited to 2^32 - 64 cells per solver");
   if ((long long)c.nx * c.ny * c.nz > (1LL << 34)) return fail(DTM_ERR_LIMIT, "grid too large");
   if (!(c.dx > 0 && c.dy > 0 && c.dz > 0 && c.dt > 0)) return fail(DTM_ERR_ARG, "steps must be positive");
   const double cour = c.dt * sqrt(1 / (c.dx * c.dx) + 1 / (c.dy * c.dy) + 1 / (c.dz * c.dz));
@@ -692,6 +717,9 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   if (c.nsoft < 0 || c.nsoft > DTM_MAX_SOFT) return fail(DTM_ERR_ARG, "nsoft out of range");
   if (c.storage != DTM_STORE_FP32 && c.storage != DTM_STORE_FP16) return fail(DTM_ERR_ARG, "bad storage");
   if (c.sweep != DTM_SWEEP_TWOPASS && c.sweep != DTM_SWEEP_FUSED) return fail(DTM_ERR_ARG, "bad sweep");
+  if (c.pml_kind != DTM_PML_CPML && c.pml_kind != DTM_PML_SPLIT) return fail(DTM_ERR_ARG, "bad pml_kind");
+  if (c.pml_kind == DTM_PML_SPLIT && c.sweep == DTM_SWEEP_FUSED)
+    return fail(DTM_ERR_ARG, "the split-field layer runs with the two-pass sweeps");
   const int n3[3] = {c.nx, c.ny, c.nz};
   for (int a = 0; a < 3; a++) {
     if (c.npml[2 * a] < 0 || c.npml[2 * a + 1] < 0 || c.npml[2 * a] + c.npml[2 * a + 1] > n3[a])
@@ -750,14 +778,15 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
     }
     g.zero = z;
   }
+  g.zidx = (unsigned)((size_t)c.nx * c.ny * c.nz);
   const size_t n = (size_t)c.nx * c.ny * c.nz;
   const size_t esz = s->store ? 2 : 4;
   int rc = 0;
   for (int q = 0; q < 3 && !rc; q++) {
-    rc = sdalloc(s, &s->E[q], esz * n);
-    if (!rc) rc = sdalloc(s, &s->H[q], esz * n);
-    if (!rc && s->torre) rc = sdalloc(s, &s->E2[q], esz * n);
-    if (!rc && s->torre) rc = sdalloc(s, &s->H2[q], esz * n);
+    rc = sdalloc(s, &s->E[q], esz * (n + kPad));
+    if (!rc) rc = sdalloc(s, &s->H[q], esz * (n + kPad));
+    if (!rc && s->torre) rc = sdalloc(s, &s->E2[q], esz * (n + kPad));
+    if (!rc && s->torre) rc = sdalloc(s, &s->H2[q], esz * (n + kPad));
   }
   if (!rc) rc = setup_cpml(s);
   if (!rc) rc = setup_tfsf(s);

@@ -203,9 +203,11 @@ def test_cpml_next_to_interior_boundary(npml):
     check_fields(g, o)
 
 
-def test_fp16_storage_tracks_fp32():
-    w32 = configs.plane_wave_sphere(nx=64, ny=48, nz=48, npml=8, inset=4, radius=8, steps=300, periods_w=1.0, periods_t0=3.0)
-    w16 = configs.plane_wave_sphere(nx=64, ny=48, nz=48, npml=8, inset=4, radius=8, steps=300,
+@pytest.mark.parametrize("nz", [48, 47])  # even nz: two cells per lane; odd: one
+def test_fp16_storage_tracks_fp32(nz):
+    w32 = configs.plane_wave_sphere(nx=64, ny=48, nz=nz, npml=8, inset=4, radius=8, steps=300, periods_w=1.0,
+                                    periods_t0=3.0)
+    w16 = configs.plane_wave_sphere(nx=64, ny=48, nz=nz, npml=8, inset=4, radius=8, steps=300,
                                     storage=abi.STORE_FP16, periods_w=1.0, periods_t0=3.0)
     a = configs.apply(w32, solver(w32.cfg))
     b = configs.apply(w16, solver(w16.cfg))
@@ -214,3 +216,27 @@ def test_fp16_storage_tracks_fp32():
     err = rel_l2(b.fields(), a.fields())
     print("fp16 vs fp32 rel-L2", err)
     assert err < 2e-2
+
+
+@pytest.mark.parametrize("inset", [4, 0])  # 0: the TF/SF bands reach into the layer
+def test_split_field_layer_plane_wave(inset, sweep):
+    """The reference's split-field absorbing layer (genAlg.py:117-148)."""
+    w = configs.plane_wave_sphere(nx=56, ny=40, nz=40, npml=8, inset=inset, radius=8, steps=300, periods_w=1.0,
+                                  periods_t0=3.0)
+    w.cfg.pml_kind = abi.PML_SPLIT
+    if sweep == abi.SWEEP_FUSED:  # the split layer runs with the two-pass sweeps only
+        with pytest.raises(abi.DTMError):
+            solver(w.cfg)
+        return
+    g, o = run_pair(w)
+    check_fields(g, o)
+
+
+def test_split_field_layer_dipole(sweep):
+    w = configs.vacuum_dipole(n=40, npml=8, steps=300)
+    w.cfg.pml_kind = abi.PML_SPLIT
+    if sweep == abi.SWEEP_FUSED:
+        return
+    g, o = run_pair(w)
+    check_fields(g, o)
+    assert rel_l2([g.probes()], [o.probes()]) <= TOL

@@ -1,0 +1,56 @@
+"""Purcell factor of the dispersive dipole run (Drude + Lorentz sphere, the
+reference's DipoleTFSF with the scattered field inside the box): FP32 HIP
+path against the FP64 oracle to the stated Purcell bound 1e-3 (the scattered
+field at the dipole is far below the grid's field scale, so its FP32 error is
+well above the 1e-5 field parity), and FP16 storage against FP32.  Error
+measure: max_w |P_a(w) - P_b(w)| / max_w |P_b(w)|.  FP16 stores the total
+field just outside the small TF/SF box, where the dipole's near field is
+largest, and the box corrections subtract the FP64 incident field from it:
+the stored rounding error lands on the scattered field the Purcell factor is
+made of, every step, and it accumulates over the run (measured here: 2e-2
+after 600 steps, 0.27 after 1500).  The FP16 check is therefore a sanity
+bound (finite, same sign structure); bench.py reports the full-size error."""
+import numpy as np
+import pytest
+
+from dtmaxwell4_amd import abi, configs, purcell
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def purcell_curve(sim, w):
+    FE = sim.dft()[0]
+    Fd = purcell.dipole_spectrum(w.cfg.tfsf, w.omegas, w.steps, w.cfg.dt)
+    return purcell.purcell_factor(w.omegas, FE, Fd)
+
+
+def perr(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def small(storage=abi.STORE_FP32):
+    return configs.drude_lorentz_purcell(n=48, npml=8, radius=8, gap=3, nfreq=16, steps=1500,
+                                         cells_per_lambda=20, storage=storage)
+
+
+def test_purcell_fp32_matches_fp64_oracle_and_fp16_tracks_fp32():
+    from dtmaxwell4_amd.solver import Solver
+    w32 = small()
+    g = configs.apply(w32, Solver(w32.cfg))
+    g.step(w32.steps)
+    P32 = purcell_curve(g, w32)
+    o = configs.apply(w32, Oracle(w32.cfg))
+    o.step(w32.steps)
+    P64 = purcell_curve(o, w32)
+    e32 = perr(P32, P64)
+    print("P range", P64.min(), P64.max(), "fp32 vs fp64", e32)
+    assert np.ptp(P64) > 1e-3          # the sphere changes P measurably
+    assert e32 < 1e-3
+    w16 = small(abi.STORE_FP16)
+    h = configs.apply(w16, Solver(w16.cfg))
+    h.step(w16.steps)
+    P16 = purcell_curve(h, w16)
+    e16 = perr(P16, P32)
+    print("fp16 vs fp32", e16)
+    assert np.isfinite(P16).all() and e16 < 1.0
