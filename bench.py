@@ -40,6 +40,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel_prefix, sweep):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by a plain copy; the file names
+    its command): read MB + write MB of the first row whose kernel name starts
+    with kernel_prefix, or None."""
+    path = os.path.join(ROOT, "profiles", f"k_{'torre_fused' if sweep else 'update_twopass'}_fp32_512x256x256.txt")
+    try:
+        for line in open(path):
+            if line.startswith(kernel_prefix):
+                f = line[70:].split()
+                return dict(bytes=(float(f[2]) + float(f[3])) * 1e6, source=os.path.relpath(path, ROOT))
+    except OSError:
+        return None
+    return None
+
+
 def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
     import numpy as np
 
@@ -242,6 +258,10 @@ def main():
             log(f"[bench] cpu baseline failed: {e}")
     dom = r32["groups"][r32["dominant"]]
     achieved = dom["gbs"]
+    # measured HBM bytes of the dominant kernel, from the committed profile
+    prefix = {"E interior": "void dtm::k_update_e<0, 1,", "H interior": "void dtm::k_update_h<0, 1,",
+              "step interior": "void dtm::k_torre<0, false>"}.get(r32["dominant"], "")
+    tr = profiled_traffic(prefix, sw) if prefix and (nx, ny, nz) == (512, 256, 256) else None
     out = {
         "metric": "Gcells/s at 512x256x256 FP32 vs FP16-store; adjoint/forward time ratio",
         "value": round(r32["gcells"], 3),
@@ -259,7 +279,9 @@ def main():
                                "TF/SF box 16 cells inside the CPML",
                    "grid": [nx, ny, nz], "storage": "fp32", "kernel": r32["kernel"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if tr is None else round(tr["bytes"]),
+                     "traffic_source": None if tr is None else tr["source"],
                      "kernel": r32["kernel"] + " " + r32["dominant"], "avg_launch_ms": dom["avg_ms"],
                      "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "cells_per_launch": dom["cells"]},
         "effective_yee_gbs": round(r32["gcells"] * 72, 1),
