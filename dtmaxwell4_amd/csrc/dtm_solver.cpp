@@ -521,26 +521,18 @@ static int setup_cpml(dtm_solver* s) {
   return 0;
 }
 
-// Interior box: every x/y stencil neighbour exists and no x/y CPML; edges
-// take the rest with full boundary handling.  The fused step recomputes H
-// one cell below and two cells above the E box it owns (x planes and y rows)
-// and reads E three cells beyond it, so the interior stops one cell short of
-// the low CPML slab and two short of the high one: its recomputed H then
-// never needs the x/y CPML terms the interior kernel leaves out.
+// Fused step: the main launch spans x (x boundary rule and x CPML per
+// plane) and the y rows whose recomputed H halo (one row below, two above the
+// owned rows) stays outside the y CPML slabs; the two y strips take the rest
+// with the y CPML.
 static void setup_regions(dtm_solver* s) {
   const dtm_config& c = s->cfg;
-  auto edge = [&](int a, int lo_or_hi) {
-    const int n = c.npml[2 * a + lo_or_hi];
-    return std::max(n > 0 ? n + 1 + lo_or_hi : 0, 3);
-  };
-  int x0 = std::min(edge(0, 0), c.nx), x1 = std::max(c.nx - edge(0, 1), x0);
-  int y0 = std::min(edge(1, 0), c.ny), y1 = std::max(c.ny - edge(1, 1), y0);
-  s->main = Region{x0, x1, y0, y1, 0, c.nz};
+  const int ylo = c.npml[2] > 0 ? std::min(c.npml[2] + 1, c.ny) : 0;
+  const int yhi = c.npml[3] > 0 ? std::max(c.ny - c.npml[3] - 2, ylo) : c.ny;
+  s->main = Region{0, c.nx, ylo, yhi, 0, c.nz};
   s->nedge = 0;
-  s->edge[s->nedge++] = Region{0, x0, 0, c.ny, 0, c.nz};
-  s->edge[s->nedge++] = Region{x1, c.nx, 0, c.ny, 0, c.nz};
-  s->edge[s->nedge++] = Region{x0, x1, 0, y0, 0, c.nz};
-  s->edge[s->nedge++] = Region{x0, x1, y1, c.ny, 0, c.nz};
+  if (ylo > 0) s->edge[s->nedge++] = Region{0, c.nx, 0, ylo, 0, c.nz};
+  if (yhi < c.ny) s->edge[s->nedge++] = Region{0, c.nx, yhi, c.ny, 0, c.nz};
   // Two-pass sweeps: the main launch spans x (its x boundary and x CPML are
   // per plane) and leaves the y CPML slabs to the edge launch.
   const int yl = std::min(c.npml[2], c.ny), yh = std::max(c.ny - c.npml[3], yl);

@@ -9,23 +9,28 @@
 // block slides along x and the E update trails the H update by two planes,
 // the reach of the (2,4) stencil.
 //
-// Block = 64 lanes (z) x TR=16 rows (y).  Thread (ty, tx) computes H at
-// (j0-1+ty, k0-1+tx) on every plane and, for the owned interior
-// ty in [1,14), tx in [1,62), E at (j0-1+ty, k0-1+tx) two planes behind.
-// Halo H values are recomputed by neighbouring blocks from the same old
-// fields, so fields and CPML psi are double-buffered (read old, write new)
-// and blocks need no synchronisation with each other.
+// Block = 64 lanes (z) x TR=16 rows (y).  On every plane p the block
+// computes H on a 16 x 67 tile: lane tx at column k0+tx, lanes 0..2 also at
+// the halo columns k0-1, k0+64, k0+65; rows j0-1 .. j0+14.  Two planes behind
+// (m = p-2) it computes E on the owned 13 x 64 tile (rows j0 .. j0+12, every
+// lane).  Halo H values are recomputed by neighbouring blocks from the same
+// old fields, so fields and CPML psi are double-buffered (read old, write
+// new) and blocks need no synchronisation with each other.
 //
-// LDS: the E tile of the current plane (19 x 67 per component, 2 slots) for
-// the y/z stencil of H, and a 4-slot ring of the 16 x 64 H planes for the
-// y/z stencil of E.  One LDS-only barrier per plane; the next plane's global
-// loads are issued before it.
+// Pipeline: the operands of plane p+1 are loaded while plane p computes
+// (PF = 1: the block holds 1024 lanes and 157 KB of LDS, one block per CU,
+// so 128 VGPRs is the budget and a second operand set would spill), with
+// unconditional loads (positions outside the grid read the zero pad); the
+// x neighbours come from LDS rings of E tiles (Ex: 4 planes, Ey/Ez: 5) and
+// of new H tiles (5 planes); the H read-modify-write operands and E class
+// bytes pass through per-thread LDS slots; one LDS-only barrier per plane.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "dtm_internal.h"
 
@@ -35,17 +40,26 @@ namespace {
 
 constexpr int TZ = 64;            // lanes per row (one wavefront)
 constexpr int TR = 16;            // thread rows
-constexpr int OZ = TZ - 3;        // owned z per tile (61)
+constexpr int OZ = TZ;            // owned z per tile (64)
 constexpr int OR = TR - 3;        // owned rows per tile (13)
+constexpr int HC = TZ + 3;        // H tile columns (67): k0-1 .. k0+65
 constexpr int ER = TR + 3;        // E tile rows (19): j0-3 .. j0+15
-constexpr int EC = TZ + 3;        // E tile cols (67): k0-3 .. k0+63
-constexpr int ETILE = ER * EC;    // 1273
+constexpr int EC = TZ + 6;        // E tile cols (70): k0-3 .. k0+66
+constexpr int ETILE = ER * EC;    // 1330
 constexpr int NXCD = 8;
+constexpr int PF = 1;             // prefetch depth (planes); 2 spills at 128 VGPRs
 
-__device__ __forceinline__ float ld(const float* F, long long q, float) { return F[q]; }
-__device__ __forceinline__ float ld(const __half* F, long long q, float inv) { return __half2float(F[q]) * inv; }
+template <typename T> __device__ __forceinline__ float cv(T v, float inv);
+template <> __device__ __forceinline__ float cv<float>(float v, float) { return v; }
+template <> __device__ __forceinline__ float cv<__half>(__half v, float inv) { return __half2float(v) * inv; }
 __device__ __forceinline__ void st(float* F, long long q, float v, float) { F[q] = v; }
 __device__ __forceinline__ void st(__half* F, long long q, float v, float s) { F[q] = __float2half_rn(v * s); }
+// word at a position under the boundary rule (off < 0 or plane < 0: zero pad)
+template <typename T>
+__device__ __forceinline__ T ldz(const T* F, long long base, int off, unsigned zidx) {
+  const unsigned idx = (base < 0 || off < 0) ? zidx : (unsigned)base + (unsigned)off;
+  return F[idx];
+}
 
 __device__ __forceinline__ int nbr(int v, int n, int per) {
   if (v >= 0 && v < n) return v;
@@ -60,12 +74,15 @@ __device__ __forceinline__ int slab_index(int v, int n, int plo, int phi) {
   if (v >= n - phi) return plo + v - (n - phi);
   return -1;
 }
-__device__ __forceinline__ long long psi_index(const Grid& g, const Cpml& p, int a, int s, int i, int j, int k) {
-  const int P = p.plo[a] + p.phi[a];
-  if (a == 0) return ((long long)s * g.ny + j) * g.nz + k;
-  if (a == 1) return ((long long)i * P + s) * g.nz + k;
-  return ((long long)i * g.ny + j) * P + s;
+// psi index (psi arrays hold far fewer than 2^32 floats)
+__device__ __forceinline__ unsigned psi_index(const Grid& g, const Cpml& p, int a, int s, int i, int j, int k) {
+  const unsigned P = (unsigned)(p.plo[a] + p.phi[a]);
+  if (a == 0) return ((unsigned)s * (unsigned)g.ny + (unsigned)j) * (unsigned)g.nz + (unsigned)k;
+  if (a == 1) return ((unsigned)i * P + (unsigned)s) * (unsigned)g.nz + (unsigned)k;
+  return ((unsigned)i * (unsigned)g.ny + (unsigned)j) * P + (unsigned)s;
 }
+// ring slot of plane p in an R-plane ring (p >= -8)
+template <int R> __device__ __forceinline__ int slot(int p) { return (p + 4 * R) % R; }
 
 struct TorreTile {
   int r, i0, i1, j0, k0;
@@ -91,22 +108,38 @@ __device__ __forceinline__ bool torre_locate(const TorreSet& ts, TorreTile& t) {
   return true;
 }
 
-// CPML psi for one derivative: read old, write new only when `own`.
-__device__ __forceinline__ void cpml_dd(const float* po, float* pn, long long pq, float b, float c, float& D,
-                                        bool own) {
-  const float v = b * po[pq] + c * D;
+// CPML psi for one derivative from the old value po: write new when `own`.
+__device__ __forceinline__ void cpml_d(float po, float* pn, unsigned pq, float b, float c, float& D, bool own) {
+  const float v = b * po + c * D;
   if (own) pn[pq] = v;
   D += v;
 }
 
+// Operands of one plane p, fetched PF planes ahead: the Ex tile values of
+// plane p and the Ey / Ez tile values of plane p+1 (two flat positions per
+// thread), the old H of plane p at this thread's main and halo columns, and
+// the class byte of plane p-2 (the E plane of that iteration).
+template <typename T> struct PreT {
+  T ex[2], ey[2], ez[2];
+  T hx, hy, hz, gx, gy, gz;
+  uint8_t cls;
+};
+
 }  // namespace
 
-template <int STORE, bool GEN>
+// GEN bit 0: x boundary rule and x CPML (per plane); bit 1: y CPML.
+template <int STORE, int GEN>
 __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fields<STORE> fn, Scales sc,
                                                   Cpml pmo, Cpml pmn, Materials mt, Ade ade, TorreSet ts) {
+  using T = typename std::conditional<STORE == 0, float, __half>::type;
+  constexpr bool GX = GEN & 1, GY = GEN & 2;
   __shared__ ClassEntry tbl[kMaxClasses];
-  __shared__ float sE[2][3][ETILE];       // E tile of the current plane
-  __shared__ float sH[4][3][TR][TZ];      // ring of new H planes
+  __shared__ float rEx[4][ETILE];         // E tile rings: Ex planes p-3..p
+  __shared__ float rEy[5][ETILE];         // Ey, Ez planes p-3..p+1
+  __shared__ float rEz[5][ETILE];
+  __shared__ float rH[5][3][TR][HC];      // new H tiles, planes p-4..p
+  __shared__ float sHo[3][TR][HC];        // this thread's old H of plane p
+  __shared__ uint8_t sCl[TR][TZ];         // this thread's E class byte of plane p-2
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * TZ + tx;
   for (int q = tid; q < mt.nclass; q += TZ * TR) tbl[q] = mt.table[q];
@@ -114,240 +147,295 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
   if (!torre_locate(ts, t)) return;  // block-uniform
   const Region& R = ts.r[t.r];
 
-  // ---- this thread's column: H at (jh, kh); E owned when inside [1,14)x[1,62)
-  const int jh = t.j0 - 1 + ty, kh = t.k0 - 1 + tx;
-  const int jm = nbr(jh, g.ny, g.per[1]), km = nbr(kh, g.nz, g.per[2]);
-  const bool hvalid = jm >= 0 && km >= 0;  // else H is the zero boundary value
-  const long long hoff = hvalid ? (long long)jm * g.sy + km : 0;
-  const bool eown = ty >= 1 && ty < TR - 2 && tx >= 1 && tx < TZ - 2 && jh >= R.j0 && jh < R.j1 && kh >= R.k0 &&
-                    kh < R.k1 && jh >= t.j0 && jh < t.j0 + OR && kh >= t.k0 && kh < t.k0 + OZ;
+  // ---- H positions of this thread: main column tx+1, halo column (lanes 0..2)
+  const int jh = t.j0 - 1 + ty;
+  const int jm = nbr(jh, g.ny, g.per[1]);
+  const int khm = t.k0 + tx;                                   // main H column
+  const int chh = tx == 0 ? 0 : (tx == 1 ? HC - 2 : HC - 1);   // halo tile column (lanes 0..2)
+  const int khh = t.k0 - 1 + chh;
+  const bool hal = tx < 3;
+  const int kmm = nbr(khm, g.nz, g.per[2]), kmh = nbr(khh, g.nz, g.per[2]);
+  const bool hv_m = jm >= 0 && kmm >= 0, hv_h = hal && jm >= 0 && kmh >= 0;
+  const int ho_m = hv_m ? jm * (int)g.sy + kmm : -1;  // in-plane offsets (ny*nz < 2^31)
+  const int ho_h = hv_h ? jm * (int)g.sy + kmh : -1;
+  // stored: rows j0..j0+12 inside the region, main column inside the region
+  const bool rown = ty >= 1 && ty < TR - 2 && jh >= R.j0 && jh < R.j1;
+  const bool hown_c = rown && khm >= R.k0 && khm < R.k1;
+  const bool eown = hown_c;  // E at (jh, khm) for the same rows and column
 
-  // ---- flat E-tile positions handled by this thread (up to two)
-  long long foff[2];
-  bool fval[2];
+  // ---- flat E-tile positions loaded by this thread (up to two)
+  int foff[2];
   int fpos[2];
   for (int u = 0; u < 2; u++) {
     const int p = tid + u * TZ * TR;
     fpos[u] = p < ETILE ? p : -1;
-    fval[u] = false;
-    foff[u] = 0;
+    foff[u] = -1;
     if (p < ETILE) {
       const int r = p / EC, c = p % EC;
       const int jj = nbr(t.j0 - 3 + r, g.ny, g.per[1]), kk = nbr(t.k0 - 3 + c, g.nz, g.per[2]);
-      if (jj >= 0 && kk >= 0) {
-        fval[u] = true;
-        foff[u] = (long long)jj * g.sy + kk;
-      }
+      if (jj >= 0 && kk >= 0) foff[u] = jj * (int)g.sy + kk;
     }
   }
+  const int ho_e = (eown && mt.cls) ? ho_m : -1;  // class byte of the owned E cell
+  const uint8_t* clsp = mt.cls ? mt.cls : static_cast<const uint8_t*>(g.zero);
+  const unsigned czidx = mt.cls ? g.zidx : 0u;
 
-  const auto* Exo = fo.E[0];
-  const auto* Eyo = fo.E[1];
-  const auto* Ezo = fo.E[2];
-  const auto* Hxo = fo.H[0];
-  const auto* Hyo = fo.H[1];
-  const auto* Hzo = fo.H[2];
+  const T* Exo = fo.E[0];
+  const T* Eyo = fo.E[1];
+  const T* Ezo = fo.E[2];
+  const T* Hxo = fo.H[0];
+  const T* Hyo = fo.H[1];
+  const T* Hzo = fo.H[2];
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
 
   auto plane = [&](int i) -> long long {  // plane base under the x boundary rule (-1: zero)
-    if (!GEN) return (long long)i * g.sx;
+    if (!GX) return (long long)i * g.sx;
     const int ii = nbr(i, g.nx, g.per[0]);
     return ii < 0 ? -1LL : (long long)ii * g.sx;
   };
-
-  // ---- x queues: Ey, Ez at planes p-2..p+1 (H stencil), Ex at p-2..p (E RMW)
-  const int p0 = t.i0 - 1;  // first H plane
-  float qy[4], qz[4], qx[3];
-  for (int s = 0; s < 4; s++) {
-    const long long pb = plane(p0 - 2 + s);
-    const bool ok = hvalid && pb >= 0;
-    qy[s] = ok ? ld(Eyo, pb + hoff, iey) : 0.f;
-    qz[s] = ok ? ld(Ezo, pb + hoff, iez) : 0.f;
-  }
-  for (int s = 0; s < 2; s++) {  // qx[2] (plane p) comes from the tile centre
-    const long long pb = plane(p0 - 2 + s);
-    qx[s] = (hvalid && pb >= 0) ? ld(Exo, pb + hoff, iex) : 0.f;
-  }
-  qx[2] = 0.f;
-  // new-H queue of this column: Hy, Hz at planes p-3..p (E stencil), Hx at p-2
-  float hyq[4] = {0.f, 0.f, 0.f, 0.f}, hzq[4] = {0.f, 0.f, 0.f, 0.f}, hxq[3] = {0.f, 0.f, 0.f};
-
-  // ---- prefetch registers for plane p: E tile values and old H
-  float pe[2][3], ph[3];
-  auto fetch = [&](int p) {
-    const long long pb = plane(p);
+  auto fetch = [&](PreT<T>& S, int p) {
+    const long long pb = plane(p), pn = plane(p + 1), pm = plane(p - 2);
+#pragma unroll
     for (int u = 0; u < 2; u++) {
-      const bool ok = fpos[u] >= 0 && fval[u] && pb >= 0;
-      pe[u][0] = ok ? ld(Exo, pb + foff[u], iex) : 0.f;
-      pe[u][1] = ok ? ld(Eyo, pb + foff[u], iey) : 0.f;
-      pe[u][2] = ok ? ld(Ezo, pb + foff[u], iez) : 0.f;
+      const int off = fpos[u] >= 0 ? foff[u] : -1;
+      S.ex[u] = ldz(Exo, pb, off, g.zidx);
+      S.ey[u] = ldz(Eyo, pn, off, g.zidx);
+      S.ez[u] = ldz(Ezo, pn, off, g.zidx);
     }
-    const bool ok = hvalid && pb >= 0;
-    ph[0] = ok ? ld(Hxo, pb + hoff, ihx) : 0.f;
-    ph[1] = ok ? ld(Hyo, pb + hoff, ihy) : 0.f;
-    ph[2] = ok ? ld(Hzo, pb + hoff, ihz) : 0.f;
+    S.hx = ldz(Hxo, pb, ho_m, g.zidx);
+    S.hy = ldz(Hyo, pb, ho_m, g.zidx);
+    S.hz = ldz(Hzo, pb, ho_m, g.zidx);
+    S.gx = ldz(Hxo, pb, ho_h, g.zidx);
+    S.gy = ldz(Hyo, pb, ho_h, g.zidx);
+    S.gz = ldz(Hzo, pb, ho_h, g.zidx);
+    S.cls = ldz(clsp, mt.cls ? pm : 0, mt.cls ? ho_e : 0, czidx);
   };
-  fetch(p0);
-
-  // per-column CPML state (z slab; x/y slabs only in GEN regions)
-  const bool pm_on = pmo.on;
-  const int zsH = (pm_on && hvalid) ? slab_index(km, g.nz, pmo.plo[2], pmo.phi[2]) : -1;
-  const int ysH = (GEN && pm_on && hvalid) ? slab_index(jm, g.ny, pmo.plo[1], pmo.phi[1]) : -1;
-  const bool design_col = mt.d_on && jh >= mt.d_j0 && jh < mt.d_j0 + mt.d_nj && kh >= mt.d_k0 &&
-                          kh < mt.d_k0 + mt.d_nk;
-  const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
-  const int pend = t.i1 + 1;  // last H plane
-
-  for (int p = p0; p <= pend; p++) {
-    const int eb = p & 1, hb = p & 3;
-    // 1) publish the E tile of plane p
+  auto ring_direct = [&](int p) {  // Ey / Ez tiles of plane p (prologue)
+    const long long pb = plane(p);
+#pragma unroll
     for (int u = 0; u < 2; u++) {
       if (fpos[u] < 0) continue;
-      sE[eb][0][fpos[u]] = pe[u][0];
-      sE[eb][1][fpos[u]] = pe[u][1];
-      sE[eb][2][fpos[u]] = pe[u][2];
+      rEy[slot<5>(p)][fpos[u]] = cv(ldz(Eyo, pb, foff[u], g.zidx), iey);
+      rEz[slot<5>(p)][fpos[u]] = cv(ldz(Ezo, pb, foff[u], g.zidx), iez);
     }
-    const float hx_old = ph[0], hy_old = ph[1], hz_old = ph[2];
-    // 2) issue the loads for plane p+1 and the queue's plane p+2
-    const long long pb = plane(p);
-    float nqy = 0.f, nqz = 0.f;
-    if (p < pend) {
-      fetch(p + 1);
-      const long long p2 = plane(p + 2);
-      if (hvalid && p2 >= 0) {
-        nqy = ld(Eyo, p2 + hoff, iey);
-        nqz = ld(Ezo, p2 + hoff, iez);
+  };
+
+  // per-column CPML state (z slab per column; y slab per row in GY launches)
+  const bool pm_on = pmo.on;
+  const int zsm = (pm_on && hv_m) ? slab_index(kmm, g.nz, pmo.plo[2], pmo.phi[2]) : -1;
+  const int zsh = (pm_on && hv_h) ? slab_index(kmh, g.nz, pmo.plo[2], pmo.phi[2]) : -1;
+  const int ysl = (GY && pm_on && jm >= 0) ? slab_index(jm, g.ny, pmo.plo[1], pmo.phi[1]) : -1;
+  const float bzm = zsm >= 0 ? pmo.bH[2][kmm] : 0.f, czm = zsm >= 0 ? pmo.cH[2][kmm] : 0.f;
+  const float bzh = zsh >= 0 ? pmo.bH[2][kmh] : 0.f, czh = zsh >= 0 ? pmo.cH[2][kmh] : 0.f;
+  const float bze = zsm >= 0 ? pmo.bE[2][kmm] : 0.f, cze = zsm >= 0 ? pmo.cE[2][kmm] : 0.f;
+  const float byh = ysl >= 0 ? pmo.bH[1][jm] : 0.f, cyh = ysl >= 0 ? pmo.cH[1][jm] : 0.f;
+  const float bye = ysl >= 0 ? pmo.bE[1][jm] : 0.f, cye = ysl >= 0 ? pmo.cE[1][jm] : 0.f;
+  const bool design_col = mt.d_on && jh >= mt.d_j0 && jh < mt.d_j0 + mt.d_nj && khm >= mt.d_k0 &&
+                          khm < mt.d_k0 + mt.d_nk;
+  const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
+  const int p0 = t.i0 - 1, pend = t.i1 + 1;  // H planes p0 .. pend
+
+  PreT<T> S[PF];
+#pragma unroll
+  for (int u = 0; u < PF; u++) {
+    S[u] = PreT<T>{};
+  }
+  ring_direct(p0 - 2);
+  ring_direct(p0 - 1);
+  ring_direct(p0);
+#pragma unroll
+  for (int u = 0; u < PF; u++) fetch(S[u], min(p0 + u, pend));
+
+  // H at tile column c (row ty) of plane p from the E rings; CPML from the
+  // old psi (written new only when `own`); returns the three new components.
+  auto h_at = [&](int p, int c, int ip, int km, int zs, float bz, float cz, bool valid, bool own, float hx0,
+                  float hy0, float hz0, float pz0, float pz1, float py0, float py1, float& hx, float& hy,
+                  float& hz) {
+    if (!valid) {
+      hx = hy = hz = 0.f;
+      return;
+    }
+    const int e = (ty + 2) * EC + (c + 2);  // this H position in the E tile
+    const float* tEx = rEx[slot<4>(p)];
+    const float* tEy = rEy[slot<5>(p)];
+    const float* tEz = rEz[slot<5>(p)];
+    const float ex0 = tEx[e];
+    float dyEz = rd1 * d24(tEz[e - 2 * EC], tEz[e - EC], tEz[e], tEz[e + EC]);
+    float dyEx = rd1 * d24(tEx[e - 2 * EC], tEx[e - EC], ex0, tEx[e + EC]);
+    float dzEy = rd2 * d24(tEy[e - 2], tEy[e - 1], tEy[e], tEy[e + 1]);
+    float dzEx = rd2 * d24(tEx[e - 2], tEx[e - 1], ex0, tEx[e + 1]);
+    float dxEz = rd0 * d24(rEz[slot<5>(p - 2)][e], rEz[slot<5>(p - 1)][e], tEz[e], rEz[slot<5>(p + 1)][e]);
+    float dxEy = rd0 * d24(rEy[slot<5>(p - 2)][e], rEy[slot<5>(p - 1)][e], tEy[e], rEy[slot<5>(p + 1)][e]);
+    if (zs >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
+      const unsigned pq = psi_index(g, pmo, 2, zs, ip, jm, km);
+      cpml_d(pz0, pmn.psiH[2][0], pq, bz, cz, dzEy, own);
+      cpml_d(pz1, pmn.psiH[2][1], pq, bz, cz, dzEx, own);
+    }
+    if (GX && pm_on) {
+      const int xs = slab_index(ip, g.nx, pmo.plo[0], pmo.phi[0]);
+      if (xs >= 0) {  // x axis (read in place: a few planes): slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
+        const float bb = pmo.bH[0][ip], cc = pmo.cH[0][ip];
+        const unsigned pq = psi_index(g, pmo, 0, xs, ip, jm, km);
+        cpml_d(pmo.psiH[0][0][pq], pmn.psiH[0][0], pq, bb, cc, dxEz, own);
+        cpml_d(pmo.psiH[0][1][pq], pmn.psiH[0][1], pq, bb, cc, dxEy, own);
       }
     }
-    // E operands for plane m = p-2 (loaded early, used after the H half)
+    if (GY && ysl >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
+      const unsigned pq = psi_index(g, pmo, 1, ysl, ip, jm, km);
+      cpml_d(py0, pmn.psiH[1][0], pq, byh, cyh, dyEx, own);
+      cpml_d(py1, pmn.psiH[1][1], pq, byh, cyh, dyEz, own);
+    }
+    hx = hx0 - dt * (dyEz - dzEy);
+    hy = hy0 - dt * (dzEx - dxEz);
+    hz = hz0 - dt * (dxEy - dyEx);
+  };
+
+  auto step = [&](int p, PreT<T>& Q) {
     const int m = p - 2;
+    // 1) publish Ex of plane p, Ey / Ez of plane p+1, the old H, the E class
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (fpos[u] < 0) continue;
+      rEx[slot<4>(p)][fpos[u]] = cv(Q.ex[u], iex);
+      rEy[slot<5>(p + 1)][fpos[u]] = cv(Q.ey[u], iey);
+      rEz[slot<5>(p + 1)][fpos[u]] = cv(Q.ez[u], iez);
+    }
+    sHo[0][ty][tx + 1] = cv(Q.hx, ihx);
+    sHo[1][ty][tx + 1] = cv(Q.hy, ihy);
+    sHo[2][ty][tx + 1] = cv(Q.hz, ihz);
+    if (hal) {
+      sHo[0][ty][chh] = cv(Q.gx, ihx);
+      sHo[1][ty][chh] = cv(Q.gy, ihy);
+      sHo[2][ty][chh] = cv(Q.gz, ihz);
+    }
+    sCl[ty][tx] = Q.cls;
+    // 2) psi and design operands of this plane, before the prefetch
+    const long long pb = plane(p);
+    const int ip = GX ? nbr(p, g.nx, g.per[0]) : p;
+    const bool hplane = pb >= 0;
+    const int im = GX ? nbr(m, g.nx, g.per[0]) : m;
     const bool edo = eown && m >= t.i0 && m < t.i1;
-    int cls = 0;
-    if (edo && mt.cls) cls = mt.cls[(long long)m * g.sx + hoff];
+    float pzm0 = 0.f, pzm1 = 0.f, pzh0 = 0.f, pzh1 = 0.f, pym0 = 0.f, pym1 = 0.f, pyh0 = 0.f, pyh1 = 0.f;
+    float pze0 = 0.f, pze1 = 0.f, pye0 = 0.f, pye1 = 0.f, dv = 0.f;
+    if (hplane && zsm >= 0) {
+      const unsigned pq = psi_index(g, pmo, 2, zsm, ip, jm, kmm);
+      pzm0 = pmo.psiH[2][0][pq];
+      pzm1 = pmo.psiH[2][1][pq];
+    }
+    if (hplane && zsh >= 0) {
+      const unsigned pq = psi_index(g, pmo, 2, zsh, ip, jm, kmh);
+      pzh0 = pmo.psiH[2][0][pq];
+      pzh1 = pmo.psiH[2][1][pq];
+    }
+    if (GY && hplane && ysl >= 0 && hv_m) {
+      const unsigned pq = psi_index(g, pmo, 1, ysl, ip, jm, kmm);
+      pym0 = pmo.psiH[1][0][pq];
+      pym1 = pmo.psiH[1][1][pq];
+      if (hv_h) {
+        const unsigned qh = psi_index(g, pmo, 1, ysl, ip, jm, kmh);
+        pyh0 = pmo.psiH[1][0][qh];
+        pyh1 = pmo.psiH[1][1][qh];
+      }
+    }
+    if (edo && zsm >= 0) {
+      const unsigned pq = psi_index(g, pmo, 2, zsm, im, jm, kmm);
+      pze0 = pmo.psiE[2][0][pq];
+      pze1 = pmo.psiE[2][1][pq];
+    }
+    if (GY && edo && ysl >= 0) {
+      const unsigned pq = psi_index(g, pmo, 1, ysl, im, jm, kmm);
+      pye0 = pmo.psiE[1][0][pq];
+      pye1 = pmo.psiE[1][1][pq];
+    }
+    const bool over = edo && design_col && m >= mt.d_i0 && m < mt.d_i0 + mt.d_ni;
+    if (over)
+      dv = mt.d_inv_eps[((long long)(m - mt.d_i0) * mt.d_nj + (jh - mt.d_j0)) * mt.d_nk + (khm - mt.d_k0)];
+    // 3) prefetch plane p+PF (clamped: unconditional)
+    fetch(Q, min(p + PF, pend));
     lds_barrier();
 
-    // 3) H at plane p (every thread; stored when owned)
-    const int c0 = (ty + 2) * EC + (tx + 2);  // this column in the E tile
-    const float* tEx = sE[eb][0];
-    const float* tEy = sE[eb][1];
-    const float* tEz = sE[eb][2];
-    const float ex0 = tEx[c0];
-    qx[2] = ex0;
-    float hxn = 0.f, hyn = 0.f, hzn = 0.f;
-    const bool hplane_ok = hvalid && pb >= 0;
-    if (hplane_ok) {
-      float dyEz = rd1 * d24(tEz[c0 - 2 * EC], tEz[c0 - EC], qz[2], tEz[c0 + EC]);
-      float dyEx = rd1 * d24(tEx[c0 - 2 * EC], tEx[c0 - EC], ex0, tEx[c0 + EC]);
-      float dzEy = rd2 * d24(tEy[c0 - 2], tEy[c0 - 1], qy[2], tEy[c0 + 1]);
-      float dzEx = rd2 * d24(tEx[c0 - 2], tEx[c0 - 1], ex0, tEx[c0 + 1]);
-      float dxEz = rd0 * d24(qz[0], qz[1], qz[2], qz[3]);
-      float dxEy = rd0 * d24(qy[0], qy[1], qy[2], qy[3]);
-      const int ip = GEN ? nbr(p, g.nx, g.per[0]) : p;
-      const bool hown = eown && p >= t.i0 && p < t.i1;
-      if (zsH >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
-        const float bb = pmo.bH[2][km], cc = pmo.cH[2][km];
-        const long long pq = psi_index(g, pmo, 2, zsH, ip, jm, km);
-        cpml_dd(pmo.psiH[2][0], pmn.psiH[2][0], pq, bb, cc, dzEy, hown);
-        cpml_dd(pmo.psiH[2][1], pmn.psiH[2][1], pq, bb, cc, dzEx, hown);
-      }
-      if (GEN && pm_on) {
-        const int xs = slab_index(ip, g.nx, pmo.plo[0], pmo.phi[0]);
-        if (xs >= 0) {  // x axis: slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
-          const float bb = pmo.bH[0][ip], cc = pmo.cH[0][ip];
-          const long long pq = psi_index(g, pmo, 0, xs, ip, jm, km);
-          cpml_dd(pmo.psiH[0][0], pmn.psiH[0][0], pq, bb, cc, dxEz, hown);
-          cpml_dd(pmo.psiH[0][1], pmn.psiH[0][1], pq, bb, cc, dxEy, hown);
-        }
-        if (ysH >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
-          const float bb = pmo.bH[1][jm], cc = pmo.cH[1][jm];
-          const long long pq = psi_index(g, pmo, 1, ysH, ip, jm, km);
-          cpml_dd(pmo.psiH[1][0], pmn.psiH[1][0], pq, bb, cc, dyEx, hown);
-          cpml_dd(pmo.psiH[1][1], pmn.psiH[1][1], pq, bb, cc, dyEz, hown);
-        }
-      }
-      hxn = hx_old - dt * (dyEz - dzEy);
-      hyn = hy_old - dt * (dzEx - dxEz);
-      hzn = hz_old - dt * (dxEy - dyEx);
-      if (hown) {
-        const long long q = pb + hoff;
-        st(fn.H[0], q, hxn, sc.sh[0]);
-        st(fn.H[1], q, hyn, sc.sh[1]);
-        st(fn.H[2], q, hzn, sc.sh[2]);
-      }
+    // 4) H at plane p: main column, then the halo column (lanes 0..2)
+    const bool hown = hown_c && p >= t.i0 && p < t.i1;
+    float hx, hy, hz;
+    h_at(p, tx + 1, ip, kmm, zsm, bzm, czm, hplane && hv_m, hown, sHo[0][ty][tx + 1], sHo[1][ty][tx + 1],
+         sHo[2][ty][tx + 1], pzm0, pzm1, pym0, pym1, hx, hy, hz);
+    const int hs = slot<5>(p);
+    rH[hs][0][ty][tx + 1] = hx;
+    rH[hs][1][ty][tx + 1] = hy;
+    rH[hs][2][ty][tx + 1] = hz;
+    if (hown) {
+      const long long q = pb + ho_m;
+      st(fn.H[0], q, hx, sc.sh[0]);
+      st(fn.H[1], q, hy, sc.sh[1]);
+      st(fn.H[2], q, hz, sc.sh[2]);
     }
-    // ring slot for plane p, and this column's queues
-    sH[hb][0][ty][tx] = hxn;
-    sH[hb][1][ty][tx] = hyn;
-    sH[hb][2][ty][tx] = hzn;
-    hyq[0] = hyq[1]; hyq[1] = hyq[2]; hyq[2] = hyq[3]; hyq[3] = hyn;
-    hzq[0] = hzq[1]; hzq[1] = hzq[2]; hzq[2] = hzq[3]; hzq[3] = hzn;
-    hxq[0] = hxq[1]; hxq[1] = hxq[2]; hxq[2] = hxn;
+    if (hal) {
+      float gx, gy, gz;
+      h_at(p, chh, ip, kmh, zsh, bzh, czh, hplane && hv_h, false, sHo[0][ty][chh], sHo[1][ty][chh],
+           sHo[2][ty][chh], pzh0, pzh1, pyh0, pyh1, gx, gy, gz);
+      rH[hs][0][ty][chh] = gx;
+      rH[hs][1][ty][chh] = gy;
+      rH[hs][2][ty][chh] = gz;
+    }
 
-    // 4) E at plane m = p-2 from the ring slot written two planes ago
+    // 5) E at plane m = p-2 (owned cells), from the H ring
     if (edo) {
-      const int mb = (p - 2) & 3;
-      const float(*rHx)[TZ] = sH[mb][0];
-      const float(*rHy)[TZ] = sH[mb][1];
-      const float(*rHz)[TZ] = sH[mb][2];
-      const float hx0 = hxq[0], hy0 = hyq[1], hz0 = hzq[1];
-      float dyHz = rd1 * d24(rHz[ty - 1][tx], hz0, rHz[ty + 1][tx], rHz[ty + 2][tx]);
-      float dyHx = rd1 * d24(rHx[ty - 1][tx], hx0, rHx[ty + 1][tx], rHx[ty + 2][tx]);
-      float dzHy = rd2 * d24(rHy[ty][tx - 1], hy0, rHy[ty][tx + 1], rHy[ty][tx + 2]);
-      float dzHx = rd2 * d24(rHx[ty][tx - 1], hx0, rHx[ty][tx + 1], rHx[ty][tx + 2]);
-      float dxHz = rd0 * d24(hzq[0], hzq[1], hzq[2], hzq[3]);
-      float dxHy = rd0 * d24(hyq[0], hyq[1], hyq[2], hyq[3]);
-      if (pm_on) {
-        const int zs = slab_index(km, g.nz, pmo.plo[2], pmo.phi[2]);
-        if (zs >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
-          const float bb = pmo.bE[2][km], cc = pmo.cE[2][km];
-          const long long pq = psi_index(g, pmo, 2, zs, m, jm, km);
-          cpml_dd(pmo.psiE[2][0], pmn.psiE[2][0], pq, bb, cc, dzHy, true);
-          cpml_dd(pmo.psiE[2][1], pmn.psiE[2][1], pq, bb, cc, dzHx, true);
+      const int c = tx + 1;  // this E cell's column in the H tile
+      const float(*h0)[TR][HC] = rH[slot<5>(m)];
+      const float hx0 = h0[0][ty][c], hy0 = h0[1][ty][c], hz0 = h0[2][ty][c];
+      float dyHz = rd1 * d24(h0[2][ty - 1][c], hz0, h0[2][ty + 1][c], h0[2][ty + 2][c]);
+      float dyHx = rd1 * d24(h0[0][ty - 1][c], hx0, h0[0][ty + 1][c], h0[0][ty + 2][c]);
+      float dzHy = rd2 * d24(h0[1][ty][c - 1], hy0, h0[1][ty][c + 1], h0[1][ty][c + 2]);
+      float dzHx = rd2 * d24(h0[0][ty][c - 1], hx0, h0[0][ty][c + 1], h0[0][ty][c + 2]);
+      float dxHz = rd0 * d24(rH[slot<5>(m - 1)][2][ty][c], hz0, rH[slot<5>(m + 1)][2][ty][c], rH[slot<5>(m + 2)][2][ty][c]);
+      float dxHy = rd0 * d24(rH[slot<5>(m - 1)][1][ty][c], hy0, rH[slot<5>(m + 1)][1][ty][c], rH[slot<5>(m + 2)][1][ty][c]);
+      if (zsm >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
+        const unsigned pq = psi_index(g, pmo, 2, zsm, im, jm, kmm);
+        cpml_d(pze0, pmn.psiE[2][0], pq, bze, cze, dzHy, true);
+        cpml_d(pze1, pmn.psiE[2][1], pq, bze, cze, dzHx, true);
+      }
+      if (GX && pm_on) {
+        const int xs = slab_index(im, g.nx, pmo.plo[0], pmo.phi[0]);
+        if (xs >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
+          const float bb = pmo.bE[0][im], cc = pmo.cE[0][im];
+          const unsigned pq = psi_index(g, pmo, 0, xs, im, jm, kmm);
+          cpml_d(pmo.psiE[0][0][pq], pmn.psiE[0][0], pq, bb, cc, dxHz, true);
+          cpml_d(pmo.psiE[0][1][pq], pmn.psiE[0][1], pq, bb, cc, dxHy, true);
         }
-        if (GEN) {
-          const int xs = slab_index(m, g.nx, pmo.plo[0], pmo.phi[0]);
-          if (xs >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
-            const float bb = pmo.bE[0][m], cc = pmo.cE[0][m];
-            const long long pq = psi_index(g, pmo, 0, xs, m, jm, km);
-            cpml_dd(pmo.psiE[0][0], pmn.psiE[0][0], pq, bb, cc, dxHz, true);
-            cpml_dd(pmo.psiE[0][1], pmn.psiE[0][1], pq, bb, cc, dxHy, true);
-          }
-          const int ys = slab_index(jm, g.ny, pmo.plo[1], pmo.phi[1]);
-          if (ys >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
-            const float bb = pmo.bE[1][jm], cc = pmo.cE[1][jm];
-            const long long pq = psi_index(g, pmo, 1, ys, m, jm, km);
-            cpml_dd(pmo.psiE[1][0], pmn.psiE[1][0], pq, bb, cc, dyHx, true);
-            cpml_dd(pmo.psiE[1][1], pmn.psiE[1][1], pq, bb, cc, dyHz, true);
-          }
-        }
+      }
+      if (GY && ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
+        const unsigned pq = psi_index(g, pmo, 1, ysl, im, jm, kmm);
+        cpml_d(pye0, pmn.psiE[1][0], pq, bye, cye, dyHx, true);
+        cpml_d(pye1, pmn.psiE[1][1], pq, bye, cye, dyHz, true);
       }
       const float cx = dyHz - dzHy, cy = dzHx - dxHz, cz = dxHy - dyHx;
-      const long long q = (long long)m * g.sx + hoff;
-      const ClassEntry& ce = tbl[cls];
+      const long long q = (long long)im * g.sx + ho_m;
+      const ClassEntry& ce = tbl[sCl[ty][tx]];
       float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
-      bool over = false;
-      if (design_col && m >= mt.d_i0 && m < mt.d_i0 + mt.d_ni) {
-        const float v = mt.d_inv_eps[((long long)(m - mt.d_i0) * mt.d_nj + (jh - mt.d_j0)) * mt.d_nk + (kh - mt.d_k0)];
-        ie0 = ie1 = ie2 = v;
-        over = true;
-      }
+      if (over) ie0 = ie1 = ie2 = dv;
       if (ce.special && !over) {  // finished by k_ade (old E -> new E)
-        const long long bq = (long long)(m - ade.i0) * ade.bsx + (long long)(jh - ade.j0) * ade.bsy + (kh - ade.k0);
+        const long long bq = (long long)(m - ade.i0) * ade.bsx + (long long)(jh - ade.j0) * ade.bsy + (khm - ade.k0);
         ade.curl[0][bq] = cx;
         ade.curl[1][bq] = cy;
         ade.curl[2][bq] = cz;
       } else {
-        st(fn.E[0], q, qx[0] + dt * ie0 * cx, sc.se[0]);
-        st(fn.E[1], q, qy[0] + dt * ie1 * cy, sc.se[1]);
-        st(fn.E[2], q, qz[0] + dt * ie2 * cz, sc.se[2]);
+        const int e = (ty + 2) * EC + (tx + 3);  // old E of this cell in the rings (plane m)
+        st(fn.E[0], q, rEx[slot<4>(m)][e] + dt * ie0 * cx, sc.se[0]);
+        st(fn.E[1], q, rEy[slot<5>(m)][e] + dt * ie1 * cy, sc.se[1]);
+        st(fn.E[2], q, rEz[slot<5>(m)][e] + dt * ie2 * cz, sc.se[2]);
       }
     }
-    // 5) advance the E queues (planes p-1 .. p+2) and Ex (p-1 .. p+1)
-    qy[0] = qy[1]; qy[1] = qy[2]; qy[2] = qy[3]; qy[3] = nqy;
-    qz[0] = qz[1]; qz[1] = qz[2]; qz[2] = qz[3]; qz[3] = nqz;
-    qx[0] = qx[1]; qx[1] = qx[2];
+  };
+
+  int p = p0;
+  for (; p + PF - 1 <= pend; p += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) step(p + u, S[u]);
   }
+#pragma unroll
+  for (int u = 0; u < PF - 1; u++)
+    if (p + u <= pend) step(p + u, S[u]);
 }
 
 // ------------------------------------------------------------ launcher
@@ -378,18 +466,50 @@ TorreSet make_torre_set(const Region* regs, int nreg, int seg) {
   return s;
 }
 
-static int torre_seg(const Region* regs, int nreg) {
-  long long cols = 0;
+// Segment length: a block computes seg + 3 H planes (one halo plane below,
+// two above) after a warm-up of about kWarm planes, and one launch runs in
+// ceil(blocks / slots) rounds; minimise rounds * (seg + 3 + kWarm).
+static int torre_seg(const Region* regs, int nreg, int slots) {
+  constexpr int kWarm = 6;
   int maxlen = 1;
   for (int t = 0; t < nreg; t++) {
     const Region& r = regs[t];
     if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
-    cols += (long long)((r.k1 - r.k0 + OZ - 1) / OZ) * ((r.j1 - r.j0 + OR - 1) / OR);
     maxlen = std::max(maxlen, r.i1 - r.i0);
   }
-  int seg = 64;
-  while (seg > 8 && cols * ((maxlen + seg - 1) / seg) < 1024) seg /= 2;
-  return seg;
+  int best = maxlen;
+  long long best_cost = -1;
+  for (int nseg = 1; nseg <= maxlen; nseg++) {
+    const int seg = (maxlen + nseg - 1) / nseg;
+    if (nseg > 1 && seg == (maxlen + nseg - 2) / (nseg - 1)) continue;
+    long long blocks = 0;
+    for (int t = 0; t < nreg; t++) {
+      const Region& r = regs[t];
+      if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
+      blocks += (long long)((r.k1 - r.k0 + OZ - 1) / OZ) * ((r.j1 - r.j0 + OR - 1) / OR) * ((r.i1 - r.i0 + seg - 1) / seg);
+    }
+    const long long cost = ((blocks + slots - 1) / slots) * (seg + 3 + kWarm);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = seg;
+    }
+    if (seg <= 4) break;
+  }
+  return best;
+}
+
+template <typename K>
+static int torre_slots(K kernel) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return 256;
+    cus = p.multiProcessorCount;
+  }
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, TZ * TR, 0) != hipSuccess || per < 1) per = 1;
+  return per * cus;
 }
 
 template <int STORE>
@@ -399,17 +519,17 @@ static void torre_go(const Grid& g, const void* fo, const void* fn, const Scales
   const Fields<STORE>& Fo = *static_cast<const Fields<STORE>*>(fo);
   const Fields<STORE>& Fn = *static_cast<const Fields<STORE>*>(fn);
   const dim3 blk(TZ, TR, 1);
+  static const int slots_m = torre_slots(k_torre<STORE, 1>);
+  static const int slots_e = torre_slots(k_torre<STORE, 3>);
   const int nm = main.i1 > main.i0 ? 1 : 0;
-  TorreSet tm = make_torre_set(&main, nm, torre_seg(&main, nm));
+  TorreSet tm = make_torre_set(&main, nm, torre_seg(&main, nm, slots_m));
   if (tm.total > 0) {
-    hipLaunchKernelGGL((k_torre<STORE, false>), dim3(tm.per * NXCD), blk, 0, st, g, Fo, Fn, sc, pmo, pmn, mt, ade,
-                       tm);
+    hipLaunchKernelGGL((k_torre<STORE, 1>), dim3(tm.per * NXCD), blk, 0, st, g, Fo, Fn, sc, pmo, pmn, mt, ade, tm);
     check_launch("k_torre<main>");
   }
-  TorreSet te = make_torre_set(regs, nreg, torre_seg(regs, nreg));
+  TorreSet te = make_torre_set(regs, nreg, torre_seg(regs, nreg, slots_e));
   if (te.total > 0) {
-    hipLaunchKernelGGL((k_torre<STORE, true>), dim3(te.per * NXCD), blk, 0, st, g, Fo, Fn, sc, pmo, pmn, mt, ade,
-                       te);
+    hipLaunchKernelGGL((k_torre<STORE, 3>), dim3(te.per * NXCD), blk, 0, st, g, Fo, Fn, sc, pmo, pmn, mt, ade, te);
     check_launch("k_torre<edge>");
   }
 }

@@ -7,8 +7,13 @@ The reference computes spectra with an external Fourier script
 
 with (A_E, ph_E) the spectrum of the scattered field at the dipole (the box
 interior is the scattered-field region, signal.hpp:153-160) and (A_d, ph_d)
-that of the dipole moment.  Here both spectra use the device DFT convention
-F(w) = sum_n f(t_n) e^{+i w t_n} dt with E sampled at t_n = (n+1) dt.
+that of the dipole moment.  The spectra it reads come from a transform with
+the kernel e^{-i w t}: the phase of the reference's own fdip.dat falls
+linearly with w (slope -0.5, minus the centre time of its pulse), where an
+e^{+i w t} transform of a pulse at positive times rises.  Here both spectra
+use the device DFT convention F(w) = sum_n f(t_n) e^{+i w t_n} dt (E sampled
+at t_n = (n+1) dt), whose values are the complex conjugates of the
+reference's: (A, ph) = (|F|, -arg F).
 """
 import numpy as np
 
@@ -37,8 +42,10 @@ def dipole_spectrum(tfsf, omegas, steps, dt):
 
 
 def purcell_factor(omegas, FE, Fd):
-    """purcell.gp:2 in complex form: 1 + (3/2) Im(conj(F_E) F_d) / (|F_d|^2 w^3)."""
+    """purcell.gp:2 on device-convention spectra: with the reference's
+    spectra conj(F), its A_E A_d sin(ph_d - ph_E) becomes Im(F_E conj(F_d)),
+    so P = 1 + (3/2) Im(F_E conj(F_d)) / (|F_d|^2 w^3)."""
     w = np.asarray(omegas, dtype=np.float64)
     FE = np.asarray(FE)
     Fd = np.asarray(Fd)
-    return 1.0 + 1.5 * np.imag(np.conj(FE) * Fd) / (np.abs(Fd) ** 2 * w ** 3)
+    return 1.0 + 1.5 * np.imag(FE * np.conj(Fd)) / (np.abs(Fd) ** 2 * w ** 3)

@@ -13,8 +13,8 @@
 * the running DFT uses sample times t = (n+1)dt for E and (n+1/2)dt for H
   and the dt weight;
 * the Purcell expression (purcell.gp:2) reproduces the closed-form Purcell
-  factor of a dipole over a perfect mirror (image dipole) under the e^{+iwt}
-  transform convention of the DFT.
+  factor of a dipole over a perfect mirror (image dipole) on spectra of the
+  reference's e^{-iwt} transform.
 """
 import json
 import os
@@ -180,15 +180,17 @@ def test_dft_uses_sample_times_and_dt_weight():
 
 def test_purcell_expression_matches_image_dipole_closed_form():
     """P = 1 + 3[sin x / x^3 - cos x / x^2], x = 2kh, for a dipole normal to a
-    perfect mirror at height h; spectra in the e^{+iwt} convention
-    (F(w) = sum f(t) e^{+iwt} dt, the convention of the device DFT)."""
+    perfect mirror at height h.  Physical amplitudes (time factor e^{-iwt})
+    are what an e^{+iwt} transform returns; purcell.gp reads spectra of the
+    e^{-iwt} transform (the reference's fdip.dat phase falls with w), which
+    are their conjugates."""
     c = 1.0
     for w, h in [(3.0, 0.2), (10.0, 0.05), (7.0, 0.31)]:
         k = w / c
         R = 2 * h
         d = 0.7 * np.exp(0.4j)                           # e^{-iwt} amplitude
         Es = 2 * d * (1 / R ** 3 - 1j * k / R ** 2) * np.exp(1j * k * R)  # image field, on axis
-        # e^{+iwt} convention: conjugate amplitudes
+        # the reference's e^{-iwt} transform: conjugate amplitudes
         Ep, dp = np.conj(Es), np.conj(d)
         P = orc.purcell(w, abs(Ep), np.angle(Ep), abs(dp), np.angle(dp))
         x = k * R

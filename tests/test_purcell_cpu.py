@@ -1,21 +1,46 @@
 """The package's Purcell post-processing (dtmaxwell4_amd/purcell.py) against
 the oracle's literal purcell.gp:2 expression and waveform, and the spectrum
 helper against a direct sum."""
+import json
+import os
+
 import numpy as np
 import pytest
 
 from dtmaxwell4_amd import abi, purcell
 from oracle import oracle as orc
 
+HERE = os.path.dirname(os.path.abspath(__file__))
+
 
 def test_purcell_factor_is_the_reference_expression():
+    """Device spectra F are the conjugates of the reference's: purcell.gp:2
+    gets (|F|, -arg F)."""
     rng = np.random.RandomState(1)
     for _ in range(50):
         w = rng.uniform(0.5, 20)
         FE = complex(*rng.normal(size=2)) * 10 ** rng.uniform(-3, 3)
         Fd = complex(*rng.normal(size=2)) * 10 ** rng.uniform(-3, 3)
-        ref = orc.purcell(w, abs(FE), np.angle(FE), abs(Fd), np.angle(Fd))
+        ref = orc.purcell(w, abs(FE), -np.angle(FE), abs(Fd), -np.angle(Fd))
         assert purcell.purcell_factor(w, FE, Fd) == pytest.approx(ref, rel=1e-12, abs=1e-12)
+
+
+def test_reference_spectra_use_the_conjugate_transform():
+    """The phase of the reference's fdip.dat falls with w (slope -0.5, the
+    pulse centre time with a minus sign): its transform kernel is e^{-iwt}.
+    The device-convention spectrum of a pulse centred at t_c rises with slope
+    +t_c."""
+    data = json.load(open(os.path.join(HERE, "golden", "purcell_rows.json")))
+    w = np.array([r["omega"] for r in data["rows"]])
+    ph = np.unwrap(np.array([r["phd"] for r in data["rows"]]))
+    slope = np.polyfit(w, ph, 1)[0]
+    assert slope == pytest.approx(-0.5, abs=0.02)
+    t = abi.Tfsf()
+    t.waveform, t.w, t.t0, t.tw, t.amp = abi.WAVE_GAUSS_SINE, 12.0, 0.5, 0.2, 1.0
+    om = np.linspace(9.0, 15.0, 13)
+    F = purcell.dipole_spectrum(t, om, 4000, 1e-3)
+    ours = np.polyfit(om, np.unwrap(np.angle(F)), 1)[0]
+    assert ours == pytest.approx(0.5, abs=0.02)
 
 
 @pytest.mark.parametrize("kind", [abi.WAVE_RAISED_COS, abi.WAVE_GAUSS_SINE])
