@@ -135,6 +135,29 @@ struct Region {  // a box of cells [i0,i1) x [j0,j1) x [k0,k1)
 
 // Regions of the fused torre kernel: per region, tiles of 61 (z) x 13 (y)
 // owned cells and x segments of `seg` planes.
+// Fused step: the z-CPML psi pointers and the slab bounds.
+struct TorrePml {
+  int on;
+  int plo[3], phi[3];
+  const float* hO[2];
+  float* hN[2];
+  const float* eO[2];
+  float* eN[2];
+};
+// Fused step: the array pointers of one step (old / new fields Ex Ey Ez Hx
+// Hy Hz, psi, materials, ADE).  The kernel reads them from device memory
+// through a pointer made opaque on every plane: they are reloaded with a few
+// scalar loads per plane instead of being held in scalar registers across the
+// plane loop (which spills them to vector-register lanes).
+struct TorreCold {
+  const void* fo[6];
+  void* fn[6];
+  TorrePml zp;
+  Cpml pmo, pmn;  // old / new psi (x and y slabs)
+  Materials mt;
+  Ade ade;
+};
+
 struct TorreSet {
   int n;
   Region r[6];
@@ -201,9 +224,11 @@ void launch_tfsf(int store, int half, const Grid& g, const void* fields, const S
 void launch_ade(int store, const Grid& g, const void* fields_src, const void* fields_dst, const Scales& sc,
                 const Materials& mt, const Ade& ade, hipStream_t st);
 // Fused time step (dtm_torre.hip): old fields/psi -> new fields/psi.
-void launch_torre(int store, const Grid& g, const void* fields_old, const void* fields_new, const Scales& sc,
-                  const Cpml& pm_old, const Cpml& pm_new, const Materials& mt, const Ade& ade, const Region* regs,
-                  int nreg, const Region& main, hipStream_t st);
+// `cold` is a device copy of the step's TorreCold (make_torre_cold).
+void launch_torre(int store, const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
+                  const Region* regs, int nreg, const Region& main, hipStream_t st);
+TorreCold make_torre_cold(const void* fields_old, const void* fields_new, int store, const Cpml& pm_old,
+                          const Cpml& pm_new, const Materials& mt, const Ade& ade);
 void launch_soft(int store, const Grid& g, const void* fields, const Scales& sc, const Soft& so,
                  hipStream_t st);
 void launch_record(int store, const Grid& g, const void* fields, const Scales& sc, const Probes& pr,

@@ -55,6 +55,12 @@ struct dtm_solver {
   Cpml pm{};         // current psi
   Cpml pmn{};        // second psi buffers (torre step)
   bool torre = false; // fused single-pass step; false: two sweeps per step
+  // device copies of the fused step's pointer table, one per buffer parity
+  // (the old / new buffers swap every step), and their host images
+  TorreCold* cold_dev[2] = {};
+  TorreCold cold_host[2];
+  bool cold_ok[2] = {false, false};
+  int cold_last = 1;
   std::vector<void*> allocs;  // everything to free
   std::vector<std::pair<void*, size_t>> state;  // field / psi buffers (zeroed by a reset)
   bool adjoint = false;  // stepping adjoint fields: no sources, no recording
@@ -941,6 +947,30 @@ static void ev_end(dtm_solver* s) {
   s->ev_used++;
 }
 
+// Device copy of the fused step's pointer table for the current buffers.
+// The two slots match the two buffer parities, so uploads happen only when a
+// pointer changes (materials, design box); the stream is drained first, as
+// a queued step may still read the slot.
+static const TorreCold* torre_cold(dtm_solver* s) {
+  const TorreCold h = make_torre_cold(fields_ptr(s), fields_next_ptr(s), s->store, s->pm, s->pmn, s->mt, s->ade);
+  for (int k = 0; k < 2; k++)
+    if (s->cold_ok[k] && memcmp(&h, &s->cold_host[k], sizeof(TorreCold)) == 0) {
+      s->cold_last = k;
+      return s->cold_dev[k];
+    }
+  const int k = s->cold_last ^ 1;
+  if (!s->cold_dev[k] && dalloc(s, &s->cold_dev[k], sizeof(TorreCold))) return nullptr;
+  if (hipStreamSynchronize(s->stream) != hipSuccess ||
+      hipMemcpy(s->cold_dev[k], &h, sizeof(TorreCold), hipMemcpyHostToDevice) != hipSuccess) {
+    fail(DTM_ERR_DEVICE, "fused-step table upload");
+    return nullptr;
+  }
+  memcpy(&s->cold_host[k], &h, sizeof(TorreCold));
+  s->cold_ok[k] = true;
+  s->cold_last = k;
+  return s->cold_dev[k];
+}
+
 static int one_step(dtm_solver* s) {
   const dtm_config& c = s->cfg;
   const long long n = s->step;
@@ -954,13 +984,13 @@ static int one_step(dtm_solver* s) {
       launch_tfsf(s->store, 2, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 0), s->bands, th, s->stream);
       ev_end(s);
     }
+    const TorreCold* cold = torre_cold(s);
+    if (!cold) return DTM_ERR_DEVICE;
     ev_begin(s, 0);
-    launch_torre(s->store, s->g, F, fields_next_ptr(s), s->sc, s->pm, s->pmn, s->mt, s->ade, nullptr, 0, s->main,
-                 s->stream);
+    launch_torre(s->store, s->g, s->sc, s->mt, cold, nullptr, 0, s->main, s->stream);
     ev_end(s);
     ev_begin(s, 1);
-    launch_torre(s->store, s->g, F, fields_next_ptr(s), s->sc, s->pm, s->pmn, s->mt, s->ade, s->edge, s->nedge, kEmpty,
-                 s->stream);
+    launch_torre(s->store, s->g, s->sc, s->mt, cold, s->edge, s->nedge, kEmpty, s->stream);
     ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, fields_next_ptr(s), s->sc, s->mt, s->ade, s->stream);

@@ -10,8 +10,9 @@
 // the reach of the (2,4) stencil.
 //
 // Block = 64 lanes (z) x TR=16 rows (y).  On every plane p the block
-// computes H on a 16 x 67 tile: lane tx at column k0+tx, lanes 0..2 also at
-// the halo columns k0-1, k0+64, k0+65; rows j0-1 .. j0+14.  Two planes behind
+// computes H on a 16 x 67 tile, rows j0-1 .. j0+14: lane tx at column
+// k0+tx, and the first wave (row 0, which owns no E) also at the halo
+// columns k0-1, k0+64, k0+65 of all 16 rows.  Two planes behind
 // (m = p-2) it computes E on the owned 13 x 64 tile (rows j0 .. j0+12, every
 // lane).  Halo H values are recomputed by neighbouring blocks from the same
 // old fields, so fields and CPML psi are double-buffered (read old, write
@@ -28,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+#include <string.h>
 
 #include <algorithm>
 #include <type_traits>
@@ -52,13 +54,25 @@ constexpr int PF = 1;             // prefetch depth (planes); 2 spills at 128 VG
 template <typename T> __device__ __forceinline__ float cv(T v, float inv);
 template <> __device__ __forceinline__ float cv<float>(float v, float) { return v; }
 template <> __device__ __forceinline__ float cv<__half>(__half v, float inv) { return __half2float(v) * inv; }
-__device__ __forceinline__ void st(float* F, long long q, float v, float) { F[q] = v; }
-__device__ __forceinline__ void st(__half* F, long long q, float v, float s) { F[q] = __float2half_rn(v * s); }
+// Pointers read from the per-plane table are generic; casting them to the
+// global address space keeps global_load/store (flat accesses would also
+// count against the LDS wait counter that every barrier drains).
+template <typename T> using GlobalT = __attribute__((address_space(1))) T;
+template <typename T> __device__ __forceinline__ GlobalT<T>* gl(T* p) { return (GlobalT<T>*)p; }
+// the table itself lives in the constant address space: scalar loads
+using ColdK = __attribute__((address_space(4))) const TorreCold;
+__device__ __forceinline__ void st(float* F, long long q, float v, float) { gl(F)[q] = v; }
+__device__ __forceinline__ void st(__half* F, long long q, float v, float s) {
+  gl(reinterpret_cast<unsigned short*>(F))[q] = __half_as_ushort(__float2half_rn(v * s));
+}
 // word at a position under the boundary rule (off < 0 or plane < 0: zero pad)
 template <typename T>
 __device__ __forceinline__ T ldz(const T* F, long long base, int off, unsigned zidx) {
   const unsigned idx = (base < 0 || off < 0) ? zidx : (unsigned)base + (unsigned)off;
-  return F[idx];
+  if constexpr (std::is_same<T, __half>::value)
+    return __ushort_as_half(gl(reinterpret_cast<const unsigned short*>(F))[idx]);
+  else
+    return gl(F)[idx];
 }
 
 __device__ __forceinline__ int nbr(int v, int n, int per) {
@@ -74,9 +88,8 @@ __device__ __forceinline__ int slab_index(int v, int n, int plo, int phi) {
   if (v >= n - phi) return plo + v - (n - phi);
   return -1;
 }
-// psi index (psi arrays hold far fewer than 2^32 floats)
-__device__ __forceinline__ unsigned psi_index(const Grid& g, const Cpml& p, int a, int s, int i, int j, int k) {
-  const unsigned P = (unsigned)(p.plo[a] + p.phi[a]);
+// psi index (psi arrays hold far fewer than 2^32 floats); P = slab cells of axis a
+__device__ __forceinline__ unsigned psi_index(const Grid& g, unsigned P, int a, int s, int i, int j, int k) {
   if (a == 0) return ((unsigned)s * (unsigned)g.ny + (unsigned)j) * (unsigned)g.nz + (unsigned)k;
   if (a == 1) return ((unsigned)i * P + (unsigned)s) * (unsigned)g.nz + (unsigned)k;
   return ((unsigned)i * (unsigned)g.ny + (unsigned)j) * P + (unsigned)s;
@@ -111,7 +124,7 @@ __device__ __forceinline__ bool torre_locate(const TorreSet& ts, TorreTile& t) {
 // CPML psi for one derivative from the old value po: write new when `own`.
 __device__ __forceinline__ void cpml_d(float po, float* pn, unsigned pq, float b, float c, float& D, bool own) {
   const float v = b * po + c * D;
-  if (own) pn[pq] = v;
+  if (own) gl(pn)[pq] = v;
   D += v;
 }
 
@@ -129,8 +142,8 @@ template <typename T> struct PreT {
 
 // GEN bit 0: x boundary rule and x CPML (per plane); bit 1: y CPML.
 template <int STORE, int GEN>
-__global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fields<STORE> fn, Scales sc,
-                                                  Cpml pmo, Cpml pmn, Materials mt, Ade ade, TorreSet ts) {
+__global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Scales sc, const uint8_t* clsg, const TorreCold* cold,
+                                                  TorreSet ts) {
   using T = typename std::conditional<STORE == 0, float, __half>::type;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
   __shared__ ClassEntry tbl[kMaxClasses];
@@ -142,22 +155,29 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
   __shared__ uint8_t sCl[TR][TZ];         // this thread's E class byte of plane p-2
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * TZ + tx;
-  for (int q = tid; q < mt.nclass; q += TZ * TR) tbl[q] = mt.table[q];
+  const ColdK* C0 = (const ColdK*)cold;  // set-up reads
+  static_assert(sizeof(ClassEntry) == 16, "class entries are copied as four words");
+  for (int q = tid; q < 4 * C0->mt.nclass; q += TZ * TR)
+    reinterpret_cast<unsigned*>(tbl)[q] = gl(reinterpret_cast<const unsigned*>(C0->mt.table))[q];
   TorreTile t;
   if (!torre_locate(ts, t)) return;  // block-uniform
   const Region& R = ts.r[t.r];
 
-  // ---- H positions of this thread: main column tx+1, halo column (lanes 0..2)
+  // ---- H positions of this thread: main column tx+1 of row ty; the first
+  // wave (row 0, which owns no E) also takes the 3 halo columns of every row,
+  // lane l at row l/3, so the other waves run one H evaluation per plane
   const int jh = t.j0 - 1 + ty;
   const int jm = nbr(jh, g.ny, g.per[1]);
-  const int khm = t.k0 + tx;                                   // main H column
-  const int chh = tx == 0 ? 0 : (tx == 1 ? HC - 2 : HC - 1);   // halo tile column (lanes 0..2)
+  const int khm = t.k0 + tx;  // main H column
+  const bool hal = ty == 0 && tx < 3 * TR;
+  const int hr = hal ? tx / 3 : 0;  // halo row
+  const int chh = tx % 3 == 0 ? 0 : (tx % 3 == 1 ? HC - 2 : HC - 1);  // halo tile column
   const int khh = t.k0 - 1 + chh;
-  const bool hal = tx < 3;
+  const int jmh = nbr(t.j0 - 1 + hr, g.ny, g.per[1]);
   const int kmm = nbr(khm, g.nz, g.per[2]), kmh = nbr(khh, g.nz, g.per[2]);
-  const bool hv_m = jm >= 0 && kmm >= 0, hv_h = hal && jm >= 0 && kmh >= 0;
+  const bool hv_m = jm >= 0 && kmm >= 0, hv_h = hal && jmh >= 0 && kmh >= 0;
   const int ho_m = hv_m ? jm * (int)g.sy + kmm : -1;  // in-plane offsets (ny*nz < 2^31)
-  const int ho_h = hv_h ? jm * (int)g.sy + kmh : -1;
+  const int ho_h = hv_h ? jmh * (int)g.sy + kmh : -1;
   // stored: rows j0..j0+12 inside the region, main column inside the region
   const bool rown = ty >= 1 && ty < TR - 2 && jh >= R.j0 && jh < R.j1;
   const bool hown_c = rown && khm >= R.k0 && khm < R.k1;
@@ -176,16 +196,10 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
       if (jj >= 0 && kk >= 0) foff[u] = jj * (int)g.sy + kk;
     }
   }
-  const int ho_e = (eown && mt.cls) ? ho_m : -1;  // class byte of the owned E cell
-  const uint8_t* clsp = mt.cls ? mt.cls : static_cast<const uint8_t*>(g.zero);
-  const unsigned czidx = mt.cls ? g.zidx : 0u;
+  const int ho_e = (eown && clsg) ? ho_m : -1;  // class byte of the owned E cell
+  const uint8_t* clsp = clsg ? clsg : static_cast<const uint8_t*>(g.zero);
+  const unsigned czidx = clsg ? g.zidx : 0u;
 
-  const T* Exo = fo.E[0];
-  const T* Eyo = fo.E[1];
-  const T* Ezo = fo.E[2];
-  const T* Hxo = fo.H[0];
-  const T* Hyo = fo.H[1];
-  const T* Hzo = fo.H[2];
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
 
@@ -194,7 +208,13 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
     const int ii = nbr(i, g.nx, g.per[0]);
     return ii < 0 ? -1LL : (long long)ii * g.sx;
   };
-  auto fetch = [&](PreT<T>& S, int p) {
+  auto fetch = [&](PreT<T>& S, int p, const ColdK* C) {
+    const T* Exo = static_cast<const T*>(C->fo[0]);
+    const T* Eyo = static_cast<const T*>(C->fo[1]);
+    const T* Ezo = static_cast<const T*>(C->fo[2]);
+    const T* Hxo = static_cast<const T*>(C->fo[3]);
+    const T* Hyo = static_cast<const T*>(C->fo[4]);
+    const T* Hzo = static_cast<const T*>(C->fo[5]);
     const long long pb = plane(p), pn = plane(p + 1), pm = plane(p - 2);
 #pragma unroll
     for (int u = 0; u < 2; u++) {
@@ -206,13 +226,17 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
     S.hx = ldz(Hxo, pb, ho_m, g.zidx);
     S.hy = ldz(Hyo, pb, ho_m, g.zidx);
     S.hz = ldz(Hzo, pb, ho_m, g.zidx);
-    S.gx = ldz(Hxo, pb, ho_h, g.zidx);
-    S.gy = ldz(Hyo, pb, ho_h, g.zidx);
-    S.gz = ldz(Hzo, pb, ho_h, g.zidx);
-    S.cls = ldz(clsp, mt.cls ? pm : 0, mt.cls ? ho_e : 0, czidx);
+    if (ty == 0) {  // wave-uniform: the halo wave
+      S.gx = ldz(Hxo, pb, ho_h, g.zidx);
+      S.gy = ldz(Hyo, pb, ho_h, g.zidx);
+      S.gz = ldz(Hzo, pb, ho_h, g.zidx);
+    }
+    S.cls = ldz(clsp, clsg ? pm : 0, clsg ? ho_e : 0, czidx);
   };
   auto ring_direct = [&](int p) {  // Ey / Ez tiles of plane p (prologue)
     const long long pb = plane(p);
+    const T* Eyo = static_cast<const T*>(C0->fo[1]);
+    const T* Ezo = static_cast<const T*>(C0->fo[2]);
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       if (fpos[u] < 0) continue;
@@ -222,17 +246,25 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
   };
 
   // per-column CPML state (z slab per column; y slab per row in GY launches)
-  const bool pm_on = pmo.on;
-  const int zsm = (pm_on && hv_m) ? slab_index(kmm, g.nz, pmo.plo[2], pmo.phi[2]) : -1;
-  const int zsh = (pm_on && hv_h) ? slab_index(kmh, g.nz, pmo.plo[2], pmo.phi[2]) : -1;
-  const int ysl = (GY && pm_on && jm >= 0) ? slab_index(jm, g.ny, pmo.plo[1], pmo.phi[1]) : -1;
-  const float bzm = zsm >= 0 ? pmo.bH[2][kmm] : 0.f, czm = zsm >= 0 ? pmo.cH[2][kmm] : 0.f;
-  const float bzh = zsh >= 0 ? pmo.bH[2][kmh] : 0.f, czh = zsh >= 0 ? pmo.cH[2][kmh] : 0.f;
-  const float bze = zsm >= 0 ? pmo.bE[2][kmm] : 0.f, cze = zsm >= 0 ? pmo.cE[2][kmm] : 0.f;
-  const float byh = ysl >= 0 ? pmo.bH[1][jm] : 0.f, cyh = ysl >= 0 ? pmo.cH[1][jm] : 0.f;
-  const float bye = ysl >= 0 ? pmo.bE[1][jm] : 0.f, cye = ysl >= 0 ? pmo.cE[1][jm] : 0.f;
-  const bool design_col = mt.d_on && jh >= mt.d_j0 && jh < mt.d_j0 + mt.d_nj && khm >= mt.d_k0 &&
-                          khm < mt.d_k0 + mt.d_nk;
+  const auto& zp = C0->zp;
+  const bool pm_on = zp.on;
+  const int xlo = zp.plo[0], xhi = zp.phi[0];
+  const unsigned Px = zp.plo[0] + zp.phi[0], Py = zp.plo[1] + zp.phi[1], Pz = zp.plo[2] + zp.phi[2];
+  const int zsm = (pm_on && hv_m) ? slab_index(kmm, g.nz, zp.plo[2], zp.phi[2]) : -1;
+  const int zsh = (pm_on && hv_h) ? slab_index(kmh, g.nz, zp.plo[2], zp.phi[2]) : -1;
+  const int ysl = (GY && pm_on && jm >= 0) ? slab_index(jm, g.ny, zp.plo[1], zp.phi[1]) : -1;
+  const int yslh = (GY && pm_on && hv_h) ? slab_index(jmh, g.ny, zp.plo[1], zp.phi[1]) : -1;
+  const auto& pc = C0->pmo;  // coefficient tables
+  const float bzm = zsm >= 0 ? gl(pc.bH[2])[kmm] : 0.f, czm = zsm >= 0 ? gl(pc.cH[2])[kmm] : 0.f;
+  const float bzh = zsh >= 0 ? gl(pc.bH[2])[kmh] : 0.f, czh = zsh >= 0 ? gl(pc.cH[2])[kmh] : 0.f;
+  const float bze = zsm >= 0 ? gl(pc.bE[2])[kmm] : 0.f, cze = zsm >= 0 ? gl(pc.cE[2])[kmm] : 0.f;
+  const float byh = ysl >= 0 ? gl(pc.bH[1])[jm] : 0.f, cyh = ysl >= 0 ? gl(pc.cH[1])[jm] : 0.f;
+  const float byhh = yslh >= 0 ? gl(pc.bH[1])[jmh] : 0.f, cyhh = yslh >= 0 ? gl(pc.cH[1])[jmh] : 0.f;
+  const float bye = ysl >= 0 ? gl(pc.bE[1])[jm] : 0.f, cye = ysl >= 0 ? gl(pc.cE[1])[jm] : 0.f;
+  const auto& mc = C0->mt;
+  const bool design_col = mc.d_on && jh >= mc.d_j0 && jh < mc.d_j0 + mc.d_nj && khm >= mc.d_k0 &&
+                          khm < mc.d_k0 + mc.d_nk;
+  const int di0 = mc.d_i0, di1 = mc.d_i0 + mc.d_ni;
   const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
   const int p0 = t.i0 - 1, pend = t.i1 + 1;  // H planes p0 .. pend
 
@@ -245,18 +277,19 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
   ring_direct(p0 - 1);
   ring_direct(p0);
 #pragma unroll
-  for (int u = 0; u < PF; u++) fetch(S[u], min(p0 + u, pend));
+  for (int u = 0; u < PF; u++) fetch(S[u], min(p0 + u, pend), C0);
 
-  // H at tile column c (row ty) of plane p from the E rings; CPML from the
-  // old psi (written new only when `own`); returns the three new components.
-  auto h_at = [&](int p, int c, int ip, int km, int zs, float bz, float cz, bool valid, bool own, float hx0,
-                  float hy0, float hz0, float pz0, float pz1, float py0, float py1, float& hx, float& hy,
-                  float& hz) {
+  // H at tile row `row`, column c of plane p (grid row jr, column km) from
+  // the E rings; CPML from the old psi (written new only when `own`); returns
+  // the three new components.
+  auto h_at = [&](const ColdK* C, int p, int row, int c, int ip, int jr, int km, int zs, float bz, float cz,
+                  int ys, float by, float cy, bool valid, bool own, float hx0, float hy0, float hz0, float pz0,
+                  float pz1, float py0, float py1, float& hx, float& hy, float& hz) {
     if (!valid) {
       hx = hy = hz = 0.f;
       return;
     }
-    const int e = (ty + 2) * EC + (c + 2);  // this H position in the E tile
+    const int e = (row + 2) * EC + (c + 2);  // this H position in the E tile
     const float* tEx = rEx[slot<4>(p)];
     const float* tEy = rEy[slot<5>(p)];
     const float* tEz = rEz[slot<5>(p)];
@@ -268,23 +301,23 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
     float dxEz = rd0 * d24(rEz[slot<5>(p - 2)][e], rEz[slot<5>(p - 1)][e], tEz[e], rEz[slot<5>(p + 1)][e]);
     float dxEy = rd0 * d24(rEy[slot<5>(p - 2)][e], rEy[slot<5>(p - 1)][e], tEy[e], rEy[slot<5>(p + 1)][e]);
     if (zs >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
-      const unsigned pq = psi_index(g, pmo, 2, zs, ip, jm, km);
-      cpml_d(pz0, pmn.psiH[2][0], pq, bz, cz, dzEy, own);
-      cpml_d(pz1, pmn.psiH[2][1], pq, bz, cz, dzEx, own);
+      const unsigned pq = psi_index(g, Pz, 2, zs, ip, jr, km);
+      cpml_d(pz0, C->zp.hN[0], pq, bz, cz, dzEy, own);
+      cpml_d(pz1, C->zp.hN[1], pq, bz, cz, dzEx, own);
     }
     if (GX && pm_on) {
-      const int xs = slab_index(ip, g.nx, pmo.plo[0], pmo.phi[0]);
+      const int xs = slab_index(ip, g.nx, xlo, xhi);
       if (xs >= 0) {  // x axis (read in place: a few planes): slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
-        const float bb = pmo.bH[0][ip], cc = pmo.cH[0][ip];
-        const unsigned pq = psi_index(g, pmo, 0, xs, ip, jm, km);
-        cpml_d(pmo.psiH[0][0][pq], pmn.psiH[0][0], pq, bb, cc, dxEz, own);
-        cpml_d(pmo.psiH[0][1][pq], pmn.psiH[0][1], pq, bb, cc, dxEy, own);
+        const float bb = gl(C->pmo.bH[0])[ip], cc = gl(C->pmo.cH[0])[ip];
+        const unsigned pq = psi_index(g, Px, 0, xs, ip, jr, km);
+        cpml_d(gl(C->pmo.psiH[0][0])[pq], C->pmn.psiH[0][0], pq, bb, cc, dxEz, own);
+        cpml_d(gl(C->pmo.psiH[0][1])[pq], C->pmn.psiH[0][1], pq, bb, cc, dxEy, own);
       }
     }
-    if (GY && ysl >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
-      const unsigned pq = psi_index(g, pmo, 1, ysl, ip, jm, km);
-      cpml_d(py0, pmn.psiH[1][0], pq, byh, cyh, dyEx, own);
-      cpml_d(py1, pmn.psiH[1][1], pq, byh, cyh, dyEz, own);
+    if (GY && ys >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
+      const unsigned pq = psi_index(g, Py, 1, ys, ip, jr, km);
+      cpml_d(py0, C->pmn.psiH[1][0], pq, by, cy, dyEx, own);
+      cpml_d(py1, C->pmn.psiH[1][1], pq, by, cy, dyEz, own);
     }
     hx = hx0 - dt * (dyEz - dzEy);
     hy = hy0 - dt * (dzEx - dxEz);
@@ -293,6 +326,10 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
 
   auto step = [&](int p, PreT<T>& Q) {
     const int m = p - 2;
+    // the cold operands are re-read on every plane, never held across planes
+    const TorreCold* Cg = cold;
+    asm volatile("" : "+s"(Cg));
+    const ColdK* C = (const ColdK*)Cg;
     // 1) publish Ex of plane p, Ey / Ez of plane p+1, the old H, the E class
 #pragma unroll
     for (int u = 0; u < 2; u++) {
@@ -305,9 +342,9 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
     sHo[1][ty][tx + 1] = cv(Q.hy, ihy);
     sHo[2][ty][tx + 1] = cv(Q.hz, ihz);
     if (hal) {
-      sHo[0][ty][chh] = cv(Q.gx, ihx);
-      sHo[1][ty][chh] = cv(Q.gy, ihy);
-      sHo[2][ty][chh] = cv(Q.gz, ihz);
+      sHo[0][hr][chh] = cv(Q.gx, ihx);
+      sHo[1][hr][chh] = cv(Q.gy, ihy);
+      sHo[2][hr][chh] = cv(Q.gz, ihz);
     }
     sCl[ty][tx] = Q.cls;
     // 2) psi and design operands of this plane, before the prefetch
@@ -319,64 +356,66 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
     float pzm0 = 0.f, pzm1 = 0.f, pzh0 = 0.f, pzh1 = 0.f, pym0 = 0.f, pym1 = 0.f, pyh0 = 0.f, pyh1 = 0.f;
     float pze0 = 0.f, pze1 = 0.f, pye0 = 0.f, pye1 = 0.f, dv = 0.f;
     if (hplane && zsm >= 0) {
-      const unsigned pq = psi_index(g, pmo, 2, zsm, ip, jm, kmm);
-      pzm0 = pmo.psiH[2][0][pq];
-      pzm1 = pmo.psiH[2][1][pq];
+      const unsigned pq = psi_index(g, Pz, 2, zsm, ip, jm, kmm);
+      pzm0 = gl(C->zp.hO[0])[pq];
+      pzm1 = gl(C->zp.hO[1])[pq];
     }
     if (hplane && zsh >= 0) {
-      const unsigned pq = psi_index(g, pmo, 2, zsh, ip, jm, kmh);
-      pzh0 = pmo.psiH[2][0][pq];
-      pzh1 = pmo.psiH[2][1][pq];
+      const unsigned pq = psi_index(g, Pz, 2, zsh, ip, jmh, kmh);
+      pzh0 = gl(C->zp.hO[0])[pq];
+      pzh1 = gl(C->zp.hO[1])[pq];
     }
     if (GY && hplane && ysl >= 0 && hv_m) {
-      const unsigned pq = psi_index(g, pmo, 1, ysl, ip, jm, kmm);
-      pym0 = pmo.psiH[1][0][pq];
-      pym1 = pmo.psiH[1][1][pq];
-      if (hv_h) {
-        const unsigned qh = psi_index(g, pmo, 1, ysl, ip, jm, kmh);
-        pyh0 = pmo.psiH[1][0][qh];
-        pyh1 = pmo.psiH[1][1][qh];
-      }
+      const unsigned pq = psi_index(g, Py, 1, ysl, ip, jm, kmm);
+      pym0 = gl(C->pmo.psiH[1][0])[pq];
+      pym1 = gl(C->pmo.psiH[1][1])[pq];
+    }
+    if (GY && hplane && yslh >= 0) {
+      const unsigned qh = psi_index(g, Py, 1, yslh, ip, jmh, kmh);
+      pyh0 = gl(C->pmo.psiH[1][0])[qh];
+      pyh1 = gl(C->pmo.psiH[1][1])[qh];
     }
     if (edo && zsm >= 0) {
-      const unsigned pq = psi_index(g, pmo, 2, zsm, im, jm, kmm);
-      pze0 = pmo.psiE[2][0][pq];
-      pze1 = pmo.psiE[2][1][pq];
+      const unsigned pq = psi_index(g, Pz, 2, zsm, im, jm, kmm);
+      pze0 = gl(C->zp.eO[0])[pq];
+      pze1 = gl(C->zp.eO[1])[pq];
     }
     if (GY && edo && ysl >= 0) {
-      const unsigned pq = psi_index(g, pmo, 1, ysl, im, jm, kmm);
-      pye0 = pmo.psiE[1][0][pq];
-      pye1 = pmo.psiE[1][1][pq];
+      const unsigned pq = psi_index(g, Py, 1, ysl, im, jm, kmm);
+      pye0 = gl(C->pmo.psiE[1][0])[pq];
+      pye1 = gl(C->pmo.psiE[1][1])[pq];
     }
-    const bool over = edo && design_col && m >= mt.d_i0 && m < mt.d_i0 + mt.d_ni;
-    if (over)
-      dv = mt.d_inv_eps[((long long)(m - mt.d_i0) * mt.d_nj + (jh - mt.d_j0)) * mt.d_nk + (khm - mt.d_k0)];
+    const bool over = edo && design_col && m >= di0 && m < di1;
+    if (over) {
+      const auto& M = C->mt;
+      dv = gl(M.d_inv_eps)[((long long)(m - M.d_i0) * M.d_nj + (jh - M.d_j0)) * M.d_nk + (khm - M.d_k0)];
+    }
     // 3) prefetch plane p+PF (clamped: unconditional)
-    fetch(Q, min(p + PF, pend));
+    fetch(Q, min(p + PF, pend), C);
     lds_barrier();
 
     // 4) H at plane p: main column, then the halo column (lanes 0..2)
     const bool hown = hown_c && p >= t.i0 && p < t.i1;
     float hx, hy, hz;
-    h_at(p, tx + 1, ip, kmm, zsm, bzm, czm, hplane && hv_m, hown, sHo[0][ty][tx + 1], sHo[1][ty][tx + 1],
-         sHo[2][ty][tx + 1], pzm0, pzm1, pym0, pym1, hx, hy, hz);
+    h_at(C, p, ty, tx + 1, ip, jm, kmm, zsm, bzm, czm, ysl, byh, cyh, hplane && hv_m, hown, sHo[0][ty][tx + 1],
+         sHo[1][ty][tx + 1], sHo[2][ty][tx + 1], pzm0, pzm1, pym0, pym1, hx, hy, hz);
     const int hs = slot<5>(p);
     rH[hs][0][ty][tx + 1] = hx;
     rH[hs][1][ty][tx + 1] = hy;
     rH[hs][2][ty][tx + 1] = hz;
     if (hown) {
       const long long q = pb + ho_m;
-      st(fn.H[0], q, hx, sc.sh[0]);
-      st(fn.H[1], q, hy, sc.sh[1]);
-      st(fn.H[2], q, hz, sc.sh[2]);
+      st(static_cast<T*>(C->fn[3]), q, hx, sc.sh[0]);
+      st(static_cast<T*>(C->fn[4]), q, hy, sc.sh[1]);
+      st(static_cast<T*>(C->fn[5]), q, hz, sc.sh[2]);
     }
     if (hal) {
       float gx, gy, gz;
-      h_at(p, chh, ip, kmh, zsh, bzh, czh, hplane && hv_h, false, sHo[0][ty][chh], sHo[1][ty][chh],
-           sHo[2][ty][chh], pzh0, pzh1, pyh0, pyh1, gx, gy, gz);
-      rH[hs][0][ty][chh] = gx;
-      rH[hs][1][ty][chh] = gy;
-      rH[hs][2][ty][chh] = gz;
+      h_at(C, p, hr, chh, ip, jmh, kmh, zsh, bzh, czh, yslh, byhh, cyhh, hplane && hv_h, false, sHo[0][hr][chh],
+           sHo[1][hr][chh], sHo[2][hr][chh], pzh0, pzh1, pyh0, pyh1, gx, gy, gz);
+      rH[hs][0][hr][chh] = gx;
+      rH[hs][1][hr][chh] = gy;
+      rH[hs][2][hr][chh] = gz;
     }
 
     // 5) E at plane m = p-2 (owned cells), from the H ring
@@ -391,23 +430,23 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
       float dxHz = rd0 * d24(rH[slot<5>(m - 1)][2][ty][c], hz0, rH[slot<5>(m + 1)][2][ty][c], rH[slot<5>(m + 2)][2][ty][c]);
       float dxHy = rd0 * d24(rH[slot<5>(m - 1)][1][ty][c], hy0, rH[slot<5>(m + 1)][1][ty][c], rH[slot<5>(m + 2)][1][ty][c]);
       if (zsm >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
-        const unsigned pq = psi_index(g, pmo, 2, zsm, im, jm, kmm);
-        cpml_d(pze0, pmn.psiE[2][0], pq, bze, cze, dzHy, true);
-        cpml_d(pze1, pmn.psiE[2][1], pq, bze, cze, dzHx, true);
+        const unsigned pq = psi_index(g, Pz, 2, zsm, im, jm, kmm);
+        cpml_d(pze0, C->zp.eN[0], pq, bze, cze, dzHy, true);
+        cpml_d(pze1, C->zp.eN[1], pq, bze, cze, dzHx, true);
       }
       if (GX && pm_on) {
-        const int xs = slab_index(im, g.nx, pmo.plo[0], pmo.phi[0]);
+        const int xs = slab_index(im, g.nx, xlo, xhi);
         if (xs >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
-          const float bb = pmo.bE[0][im], cc = pmo.cE[0][im];
-          const unsigned pq = psi_index(g, pmo, 0, xs, im, jm, kmm);
-          cpml_d(pmo.psiE[0][0][pq], pmn.psiE[0][0], pq, bb, cc, dxHz, true);
-          cpml_d(pmo.psiE[0][1][pq], pmn.psiE[0][1], pq, bb, cc, dxHy, true);
+          const float bb = gl(C->pmo.bE[0])[im], cc = gl(C->pmo.cE[0])[im];
+          const unsigned pq = psi_index(g, Px, 0, xs, im, jm, kmm);
+          cpml_d(gl(C->pmo.psiE[0][0])[pq], C->pmn.psiE[0][0], pq, bb, cc, dxHz, true);
+          cpml_d(gl(C->pmo.psiE[0][1])[pq], C->pmn.psiE[0][1], pq, bb, cc, dxHy, true);
         }
       }
       if (GY && ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
-        const unsigned pq = psi_index(g, pmo, 1, ysl, im, jm, kmm);
-        cpml_d(pye0, pmn.psiE[1][0], pq, bye, cye, dyHx, true);
-        cpml_d(pye1, pmn.psiE[1][1], pq, bye, cye, dyHz, true);
+        const unsigned pq = psi_index(g, Py, 1, ysl, im, jm, kmm);
+        cpml_d(pye0, C->pmn.psiE[1][0], pq, bye, cye, dyHx, true);
+        cpml_d(pye1, C->pmn.psiE[1][1], pq, bye, cye, dyHz, true);
       }
       const float cx = dyHz - dzHy, cy = dzHx - dxHz, cz = dxHy - dyHx;
       const long long q = (long long)im * g.sx + ho_m;
@@ -415,15 +454,16 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Fields<STORE> fo, Fiel
       float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
       if (over) ie0 = ie1 = ie2 = dv;
       if (ce.special && !over) {  // finished by k_ade (old E -> new E)
-        const long long bq = (long long)(m - ade.i0) * ade.bsx + (long long)(jh - ade.j0) * ade.bsy + (khm - ade.k0);
-        ade.curl[0][bq] = cx;
-        ade.curl[1][bq] = cy;
-        ade.curl[2][bq] = cz;
+        const auto& A = C->ade;
+        const long long bq = (long long)(m - A.i0) * A.bsx + (long long)(jh - A.j0) * A.bsy + (khm - A.k0);
+        gl(A.curl[0])[bq] = cx;
+        gl(A.curl[1])[bq] = cy;
+        gl(A.curl[2])[bq] = cz;
       } else {
         const int e = (ty + 2) * EC + (tx + 3);  // old E of this cell in the rings (plane m)
-        st(fn.E[0], q, rEx[slot<4>(m)][e] + dt * ie0 * cx, sc.se[0]);
-        st(fn.E[1], q, rEy[slot<5>(m)][e] + dt * ie1 * cy, sc.se[1]);
-        st(fn.E[2], q, rEz[slot<5>(m)][e] + dt * ie2 * cz, sc.se[2]);
+        st(static_cast<T*>(C->fn[0]), q, rEx[slot<4>(m)][e] + dt * ie0 * cx, sc.se[0]);
+        st(static_cast<T*>(C->fn[1]), q, rEy[slot<5>(m)][e] + dt * ie1 * cy, sc.se[1]);
+        st(static_cast<T*>(C->fn[2]), q, rEz[slot<5>(m)][e] + dt * ie2 * cz, sc.se[2]);
       }
     }
   };
@@ -513,32 +553,64 @@ static int torre_slots(K kernel) {
 }
 
 template <int STORE>
-static void torre_go(const Grid& g, const void* fo, const void* fn, const Scales& sc, const Cpml& pmo,
-                     const Cpml& pmn, const Materials& mt, const Ade& ade, const Region* regs, int nreg,
-                     const Region& main, hipStream_t st) {
-  const Fields<STORE>& Fo = *static_cast<const Fields<STORE>*>(fo);
-  const Fields<STORE>& Fn = *static_cast<const Fields<STORE>*>(fn);
+static void torre_go(const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
+                     const Region* regs, int nreg, const Region& main, hipStream_t st) {
   const dim3 blk(TZ, TR, 1);
   static const int slots_m = torre_slots(k_torre<STORE, 1>);
   static const int slots_e = torre_slots(k_torre<STORE, 3>);
   const int nm = main.i1 > main.i0 ? 1 : 0;
   TorreSet tm = make_torre_set(&main, nm, torre_seg(&main, nm, slots_m));
   if (tm.total > 0) {
-    hipLaunchKernelGGL((k_torre<STORE, 1>), dim3(tm.per * NXCD), blk, 0, st, g, Fo, Fn, sc, pmo, pmn, mt, ade, tm);
+    hipLaunchKernelGGL((k_torre<STORE, 1>), dim3(tm.per * NXCD), blk, 0, st, g, sc, mt.cls, cold, tm);
     check_launch("k_torre<main>");
   }
   TorreSet te = make_torre_set(regs, nreg, torre_seg(regs, nreg, slots_e));
   if (te.total > 0) {
-    hipLaunchKernelGGL((k_torre<STORE, 3>), dim3(te.per * NXCD), blk, 0, st, g, Fo, Fn, sc, pmo, pmn, mt, ade, te);
+    hipLaunchKernelGGL((k_torre<STORE, 3>), dim3(te.per * NXCD), blk, 0, st, g, sc, mt.cls, cold, te);
     check_launch("k_torre<edge>");
   }
 }
 
-void launch_torre(int store, const Grid& g, const void* fields_old, const void* fields_new, const Scales& sc,
-                  const Cpml& pm_old, const Cpml& pm_new, const Materials& mt, const Ade& ade, const Region* regs,
-                  int nreg, const Region& main, hipStream_t st) {
-  if (store == 0) torre_go<0>(g, fields_old, fields_new, sc, pm_old, pm_new, mt, ade, regs, nreg, main, st);
-  else torre_go<1>(g, fields_old, fields_new, sc, pm_old, pm_new, mt, ade, regs, nreg, main, st);
+template <int STORE>
+static void cold_fields(TorreCold& c, const void* fo, const void* fn) {
+  const Fields<STORE>& Fo = *static_cast<const Fields<STORE>*>(fo);
+  const Fields<STORE>& Fn = *static_cast<const Fields<STORE>*>(fn);
+  for (int q = 0; q < 3; q++) {
+    c.fo[q] = Fo.E[q];
+    c.fo[3 + q] = Fo.H[q];
+    c.fn[q] = Fn.E[q];
+    c.fn[3 + q] = Fn.H[q];
+  }
+}
+
+TorreCold make_torre_cold(const void* fields_old, const void* fields_new, int store, const Cpml& pm_old,
+                          const Cpml& pm_new, const Materials& mt, const Ade& ade) {
+  TorreCold c;
+  memset(&c, 0, sizeof(c));  // padding too: callers compare tables bytewise
+  if (store == 0) cold_fields<0>(c, fields_old, fields_new);
+  else cold_fields<1>(c, fields_old, fields_new);
+  c.zp.on = pm_old.on;
+  for (int a = 0; a < 3; a++) {
+    c.zp.plo[a] = pm_old.plo[a];
+    c.zp.phi[a] = pm_old.phi[a];
+  }
+  for (int sl = 0; sl < 2; sl++) {
+    c.zp.hO[sl] = pm_old.psiH[2][sl];
+    c.zp.hN[sl] = pm_new.psiH[2][sl];
+    c.zp.eO[sl] = pm_old.psiE[2][sl];
+    c.zp.eN[sl] = pm_new.psiE[2][sl];
+  }
+  memcpy(&c.pmo, &pm_old, sizeof(Cpml));
+  memcpy(&c.pmn, &pm_new, sizeof(Cpml));
+  memcpy(&c.mt, &mt, sizeof(Materials));
+  memcpy(&c.ade, &ade, sizeof(Ade));
+  return c;
+}
+
+void launch_torre(int store, const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
+                  const Region* regs, int nreg, const Region& main, hipStream_t st) {
+  if (store == 0) torre_go<0>(g, sc, mt, cold, regs, nreg, main, st);
+  else torre_go<1>(g, sc, mt, cold, regs, nreg, main, st);
 }
 
 }  // namespace dtm
