@@ -25,15 +25,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-GROUPS_TWOPASS = ("H interior", "H edge slabs", "E interior", "E edge slabs", "sources+probes")
-GROUPS_FUSED = ("step interior", "step edge slabs", "-", "-", "TF/SF+ADE+sources+probes")
+GROUPS_TWOPASS = ("H sweep", "H sweep y-CPML rows", "E sweep", "E sweep y-CPML rows", "TF/SF+ADE+sources+probes")
+GROUPS_FUSED = ("fused step", "-", "-", "-", "TF/SF+ADE+sources+probes")
 # Algorithmic bytes per cell.  Two-pass half-step sweep: read 3E+3H, write 3
 # components (+1 class byte for E).  Fused step (H and E in one pass): read
-# 3E+3H, write 3E+3H, read the class byte; cells inside the z CPML slabs also
-# read and write four FP32 psi values (two H, two E).
+# 3E+3H, write 3E+3H, read the class byte.  Each CPML slab a cell lies in adds
+# two FP32 psi read and written per half-step (16 B; 32 B per fused step).
 WORDS_PER_HALF = 9
 WORDS_FUSED = 12
-PSI_BYTES_Z = 8 * 4
+PSI_BYTES_HALF = 2 * 2 * 4
+
+
+def psi_cells(cfg):
+    """Sum over axes of the cells inside that axis' CPML slabs."""
+    n = (cfg.nx, cfg.ny, cfg.nz)
+    t = 0
+    for a in range(3):
+        o = n[(a + 1) % 3] * n[(a + 2) % 3]
+        t += (cfg.npml[2 * a] + cfg.npml[2 * a + 1]) * o
+    return t
 
 
 def log(msg):
@@ -84,7 +94,7 @@ def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
     esz = 2 if storage else 4
     fused = s.kernel_name.startswith("k_torre")
     names = GROUPS_FUSED if fused else GROUPS_TWOPASS
-    zfrac = (w.cfg.npml[4] + w.cfg.npml[5]) / nz
+    pfrac = psi_cells(w.cfg) / cells  # slab memberships per cell
     groups = {}
     for g in range(5):
         if gcnt[g] == 0:
@@ -92,9 +102,9 @@ def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
         avg = gms[g] / gcnt[g]
         gc = s.group_cells(g)
         if fused:
-            byts = gc * (WORDS_FUSED * esz + 1) + (gc * zfrac * PSI_BYTES_Z if g == 0 else 0)
+            byts = gc * (WORDS_FUSED * esz + 1 + 2 * PSI_BYTES_HALF * pfrac)
         else:
-            byts = gc * (WORDS_PER_HALF * esz + (1 if g in (2, 3) else 0))
+            byts = gc * (WORDS_PER_HALF * esz + (1 if g in (2, 3) else 0) + PSI_BYTES_HALF * pfrac)
         groups[names[g]] = dict(avg_ms=avg, launches=int(gcnt[g]), cells=int(gc),
                                 algorithmic_bytes=int(byts),
                                 gbs=(byts / (avg * 1e-3) / 1e9) if (gc and avg > 0) else None)
@@ -259,8 +269,8 @@ def main():
     dom = r32["groups"][r32["dominant"]]
     achieved = dom["gbs"]
     # measured HBM bytes of the dominant kernel, from the committed profile
-    prefix = {"E interior": "void dtm::k_update_e<0, 1,", "H interior": "void dtm::k_update_h<0, 1,",
-              "step interior": "void dtm::k_torre<0, false>"}.get(r32["dominant"], "")
+    prefix = {"E sweep": "void dtm::k_update_e<0, 3, 1,", "H sweep": "void dtm::k_update_h<0, 3, 1,",
+              "fused step": "void dtm::k_torre<0, 3>"}.get(r32["dominant"], "")
     tr = profiled_traffic(prefix, sw) if prefix and (nx, ny, nz) == (512, 256, 256) else None
     out = {
         "metric": "Gcells/s at 512x256x256 FP32 vs FP16-store; adjoint/forward time ratio",

@@ -197,7 +197,13 @@ struct Soft {
   float val[16];  // dt * inv_eps * J(t)
 };
 
-// Launchers (dtm_kernels.hip).  All enqueue on `st`.
+// Launchers (dtm_kernels.hip).  All enqueue on `st`.  `main` runs the
+// interior variant (no y CPML); `regs` run the variant with every boundary
+// rule.
+// True when a two-pass sweep runs two z cells per lane (FP16 storage, even
+// nz; for the E sweep (mt given) also no design box; never with the split
+// layer).  The paired interior variant then needs the y-CPML rows apart.
+bool sweep_pairs(int store, const Grid& g, const Cpml& pm, const Materials* mt);
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                      const Region* regs, int nreg, const Region& main, hipStream_t st);
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,

@@ -1275,6 +1275,10 @@ static bool pair_cells(int store, const Grid& g, const Materials* mt) {
   return store == 1 && g.nz % 2 == 0 && !(mt && mt->d_on);
 }
 
+bool sweep_pairs(int store, const Grid& g, const Cpml& pm, const Materials* mt) {
+  return !pm.split && pair_cells(store, g, mt);
+}
+
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                      const Region* regs, int nreg, const Region& main, hipStream_t st) {
   if (pm.split) {  // split-field layer: one cell per lane

@@ -64,11 +64,9 @@ struct dtm_solver {
   std::vector<void*> allocs;  // everything to free
   std::vector<std::pair<void*, size_t>> state;  // field / psi buffers (zeroed by a reset)
   bool adjoint = false;  // stepping adjoint fields: no sources, no recording
-  Region main{};       // fused step: interior box
-  Region edge[4]{};
-  int nedge = 0;
-  Region main2{};      // two-pass sweeps: all x, y outside the y CPML
-  Region edge2[2]{};
+  Region whole{};      // the grid: one launch of the variant with every boundary rule
+  Region main2{};      // paired two-pass sweeps: all x, y outside the y CPML
+  Region edge2[2]{};   // ... and the y-CPML rows
   int nedge2 = 0;
   // materials
   std::vector<uint8_t> cls_host;
@@ -527,20 +525,16 @@ static int setup_cpml(dtm_solver* s) {
   return 0;
 }
 
-// Fused step: the main launch spans x (x boundary rule and x CPML per
-// plane) and the y rows whose recomputed H halo (one row below, two above the
-// owned rows) stays outside the y CPML slabs; the two y strips take the rest
-// with the y CPML.
+// Launch layout.  The fused step and the one-cell-per-lane sweeps run the
+// whole grid in one launch of the variant that applies every boundary rule
+// (its x, y and z CPML branches are plane-, row- and column-uniform): at
+// 512x256x256 one launch beat an interior launch plus a y-strip launch by
+// 7-9%, because the strips' few tile columns forced short x segments.  The
+// paired FP16 sweeps keep the split: their variant with the y CPML would
+// spill, so the y-CPML rows run one cell per lane in a second launch.
 static void setup_regions(dtm_solver* s) {
   const dtm_config& c = s->cfg;
-  const int ylo = c.npml[2] > 0 ? std::min(c.npml[2] + 1, c.ny) : 0;
-  const int yhi = c.npml[3] > 0 ? std::max(c.ny - c.npml[3] - 2, ylo) : c.ny;
-  s->main = Region{0, c.nx, ylo, yhi, 0, c.nz};
-  s->nedge = 0;
-  if (ylo > 0) s->edge[s->nedge++] = Region{0, c.nx, 0, ylo, 0, c.nz};
-  if (yhi < c.ny) s->edge[s->nedge++] = Region{0, c.nx, yhi, c.ny, 0, c.nz};
-  // Two-pass sweeps: the main launch spans x (its x boundary and x CPML are
-  // per plane) and leaves the y CPML slabs to the edge launch.
+  s->whole = Region{0, c.nx, 0, c.ny, 0, c.nz};
   const int yl = std::min(c.npml[2], c.ny), yh = std::max(c.ny - c.npml[3], yl);
   s->main2 = Region{0, c.nx, yl, yh, 0, c.nz};
   s->nedge2 = 0;
@@ -987,10 +981,7 @@ static int one_step(dtm_solver* s) {
     const TorreCold* cold = torre_cold(s);
     if (!cold) return DTM_ERR_DEVICE;
     ev_begin(s, 0);
-    launch_torre(s->store, s->g, s->sc, s->mt, cold, nullptr, 0, s->main, s->stream);
-    ev_end(s);
-    ev_begin(s, 1);
-    launch_torre(s->store, s->g, s->sc, s->mt, cold, s->edge, s->nedge, kEmpty, s->stream);
+    launch_torre(s->store, s->g, s->sc, s->mt, cold, &s->whole, 1, kEmpty, s->stream);
     ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, fields_next_ptr(s), s->sc, s->mt, s->ade, s->stream);
@@ -1000,24 +991,36 @@ static int one_step(dtm_solver* s) {
       launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
   } else {
     // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
-    ev_begin(s, 0);
-    launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main2, s->stream);
-    ev_end(s);
-    ev_begin(s, 1);
-    launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge2, s->nedge2, kEmpty, s->stream);
-    ev_end(s);
+    if (sweep_pairs(s->store, s->g, s->pm, nullptr)) {
+      ev_begin(s, 0);
+      launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main2, s->stream);
+      ev_end(s);
+      ev_begin(s, 1);
+      launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge2, s->nedge2, kEmpty, s->stream);
+      ev_end(s);
+    } else {
+      ev_begin(s, 0);
+      launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, kEmpty, s->stream);
+      ev_end(s);
+    }
     if (s->tf.active) {
       ev_begin(s, 4);
       launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 0), s->bands, th, s->stream);
       ev_end(s);
     }
     // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
-    ev_begin(s, 2);
-    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main2, s->stream);
-    ev_end(s);
-    ev_begin(s, 3);
-    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge2, s->nedge2, kEmpty, s->stream);
-    ev_end(s);
+    if (sweep_pairs(s->store, s->g, s->pm, &s->mt)) {
+      ev_begin(s, 2);
+      launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main2, s->stream);
+      ev_end(s);
+      ev_begin(s, 3);
+      launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge2, s->nedge2, kEmpty, s->stream);
+      ev_end(s);
+    } else {
+      ev_begin(s, 2);
+      launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, &s->whole, 1, kEmpty, s->stream);
+      ev_end(s);
+    }
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, F, s->sc, s->mt, s->ade, s->stream);
     if (s->tf.active)
@@ -1323,16 +1326,14 @@ long long dtm_group_cells(const dtm_solver* s, int g) {
   auto cells = [](const Region& r) {
     return (long long)std::max(0, r.i1 - r.i0) * std::max(0, r.j1 - r.j0) * std::max(0, r.k1 - r.k0);
   };
-  const Region& m = s->torre ? s->main : s->main2;
-  const Region* e = s->torre ? s->edge : s->edge2;
-  const int ne = s->torre ? s->nedge : s->nedge2;
-  if (g == 0 || g == 2) return cells(m);
-  if (g == 1 || g == 3) {
-    long long t = 0;
-    for (int q = 0; q < ne; q++) t += cells(e[q]);
-    return t;
-  }
-  return 0;
+  if (g < 0 || g > 3) return 0;
+  if (s->torre) return g == 0 ? cells(s->whole) : 0;
+  const bool paired = sweep_pairs(s->store, s->g, s->pm, g < 2 ? nullptr : &s->mt);
+  if (!paired) return (g == 0 || g == 2) ? cells(s->whole) : 0;
+  if (g == 0 || g == 2) return cells(s->main2);
+  long long t = 0;
+  for (int q = 0; q < s->nedge2; q++) t += cells(s->edge2[q]);
+  return t;
 }
 
 }  // extern "C"

@@ -256,12 +256,15 @@ int dtm_time_steps(dtm_solver* s, int nsteps, double* ms);
 /* Name of the kernel variant used for the E/H sweep. */
 const char* dtm_kernel_name(const dtm_solver* s);
 /* Per-launch HIP-event timing of the sweep kernels, recorded on the solver's
- * stream while enabled.  Groups: 0 H interior, 1 H edge slabs, 2 E interior,
- * 3 E edge slabs, 4 sources+probes.  dtm_kernel_times returns the summed ms
- * and launch counts per group and resets them. */
+ * stream while enabled.  Groups: 0 H sweep (fused schedule: the whole step),
+ * 1 H sweep of the y-CPML rows, 2 E sweep, 3 E sweep of the y-CPML rows
+ * (groups 1 and 3 only with two FP16 cells per lane, whose groups 0 and 2
+ * then hold the other rows), 4 TF/SF, ADE, sources and probes.
+ * dtm_kernel_times returns the summed ms and launch counts per group and
+ * resets them. */
 int dtm_set_kernel_timing(dtm_solver* s, int on);
 int dtm_kernel_times(dtm_solver* s, double* ms, long long* count);
-/* Cells covered by group g's launches (interior box / edge slabs). */
+/* Cells covered by group g's launches. */
 long long dtm_group_cells(const dtm_solver* s, int g);
 
 /* ------------------------------------------------------- legacy surface */
