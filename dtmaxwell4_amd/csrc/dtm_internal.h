@@ -172,6 +172,7 @@ struct TfsfBands {
   int n;
   Region r[6];
   int axis[6];
+  unsigned start[7];  // first thread of each band (filled at launch)
 };
 
 struct ProbeDesc {

@@ -689,21 +689,20 @@ template <int STORE, int MODE, bool TAB>
 __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt, Ade ade,
                                               Tfsf tf, TfsfBands bands, double tinc) {
   constexpr int HALF = MODE == 1 ? 1 : 0;
-  const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // band and position from the thread index: 32-bit arithmetic (band cells
+  // number far fewer than 2^31; 64-bit division would dominate the kernel)
+  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
   int r = 0;
-  long long loc = gid;
-  for (; r < bands.n; r++) {
-    const Region& R = bands.r[r];
-    const long long cnt = (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
-    if (loc < cnt) break;
-    loc -= cnt;
-  }
+  while (r < bands.n && gid >= bands.start[r + 1]) r++;
   if (r >= bands.n) return;
   const Region& R = bands.r[r];
-  const int nk = R.k1 - R.k0, nj = R.j1 - R.j0;
-  const int k = R.k0 + (int)(loc % nk);
-  const int j = R.j0 + (int)((loc / nk) % nj);
-  const int i = R.i0 + (int)(loc / ((long long)nk * nj));
+  const unsigned nk = (unsigned)(R.k1 - R.k0), nj = (unsigned)(R.j1 - R.j0);
+  unsigned loc = gid - bands.start[r];
+  const unsigned lk = loc % nk;
+  loc /= nk;
+  const int k = R.k0 + (int)lk;
+  const int j = R.j0 + (int)(loc % nj);
+  const int i = R.i0 + (int)(loc / nj);
   auto holds = [&](const Region& Q) {
     return i >= Q.i0 && i < Q.i1 && j >= Q.j0 && j < Q.j1 && k >= Q.k0 && k < Q.k1;
   };
@@ -1315,17 +1314,24 @@ template <int STORE, int MODE>
 static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                     const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double tinc, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
+  TfsfBands b = bands;
   long long n = 0;
-  for (int r = 0; r < bands.n; r++) {
-    const Region& R = bands.r[r];
+  b.start[0] = 0;
+  for (int r = 0; r < b.n; r++) {
+    const Region& R = b.r[r];
     n += (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
+    b.start[r + 1] = (unsigned)std::min<long long>(n, 0xffffffffLL);
   }
   if (n <= 0) return;
+  if (n >= (1LL << 31)) {
+    fprintf(stderr, "dtm: TF/SF bands hold %lld cells (limit 2^31)\n", n);
+    return;
+  }
   const dim3 grid((unsigned)((n + 255) / 256));
   if (tf.tab_axis >= 0)
-    hipLaunchKernelGGL((k_tfsf<STORE, MODE, true>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, bands, tinc);
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE, true>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, b, tinc);
   else
-    hipLaunchKernelGGL((k_tfsf<STORE, MODE, false>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, bands, tinc);
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE, false>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, b, tinc);
   check_launch("k_tfsf");
 }
 
