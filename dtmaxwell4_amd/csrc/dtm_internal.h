@@ -173,6 +173,7 @@ struct TfsfBands {
   Region r[6];
   int axis[6];
   unsigned start[7];  // first thread of each band (filled at launch)
+  unsigned ovl[6];    // bit q: band q overlaps band r (filled at launch)
 };
 
 struct ProbeDesc {

@@ -150,27 +150,35 @@ __device__ __forceinline__ int comp_at(int hx, int hy, int hz) {
 // TF/SF corrections (genAlg.py:219-221) weighted by (+1,-27,+27,-1).
 // TAB: plane wave with k along one axis, the incident value is the
 // polarisation factor P[comp] times this half-step's waveform table entry.
+// The neighbours differ from the point only along a, so no face is crossed
+// unless the point lies inside the box on the other two axes; then a
+// neighbour crosses when its a-coordinate is on the other side of [hlo, hhi].
 template <bool TAB>
 __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t) {
-  const bool psf = in_sf(s, hx, hy, hz);
+  const int hp[3] = {hx, hy, hz};
+  const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+  if (hp[b1] < s.hlo[b1] || hp[b1] > s.hhi[b1] || hp[b2] < s.hlo[b2] || hp[b2] > s.hhi[b2]) return 0.f;
+  const int h = hp[a], lo = s.hlo[a], hi = s.hhi[a];
+  const bool pin = h >= lo && h <= hi;
   const int ho[4] = {-3, -1, 1, 3};
   const double wt[4] = {1.0, -27.0, 27.0, -1.0};
+  // the four neighbours (odd offsets along a) hold one component
+  const int cq = comp_at(a == 0 ? hx + 1 : hx, a == 1 ? hy + 1 : hy, a == 2 ? hz + 1 : hz);
   double acc = 0.0;
+#pragma unroll
   for (int q = 0; q < 4; q++) {
-    int h0 = hx, h1 = hy, h2 = hz;
-    if (a == 0) h0 += ho[q];
-    else if (a == 1) h1 += ho[q];
-    else h2 += ho[q];
-    const bool qsf = in_sf(s, h0, h1, h2);
-    if (qsf == psf) continue;
+    const int hq = h + ho[q];
+    const bool qin = hq >= lo && hq <= hi;
+    if (qin == pin) continue;
+    const bool qsf = s.sf_inside ? qin : !qin;  // the neighbour is in the scattered-field region
     double v;
     if (TAB) {
-      const int ha = s.tab_axis == 0 ? h0 : (s.tab_axis == 1 ? h1 : h2);
-      v = s.P[comp_at(h0, h1, h2)] * s.table[min(max(ha - s.tab_h0, 0), s.tab_n - 1)];
+      const int ha = s.tab_axis == a ? hq : hp[s.tab_axis];
+      v = s.P[cq] * s.table[min(max(ha - s.tab_h0, 0), s.tab_n - 1)];
     } else {
-      v = incident(s, comp_at(h0, h1, h2), h0, h1, h2, t);
+      v = incident(s, cq, a == 0 ? hq : hx, a == 1 ? hq : hy, a == 2 ? hq : hz, t);
     }
-    acc += wt[q] * ((qsf && !psf) ? v : -v);
+    acc += wt[q] * (qsf ? v : -v);
   }
   return (float)acc;
 }
@@ -703,18 +711,20 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
   const int k = R.k0 + (int)lk;
   const int j = R.j0 + (int)(loc % nj);
   const int i = R.i0 + (int)(loc / nj);
-  auto holds = [&](const Region& Q) {
-    return i >= Q.i0 && i < Q.i1 && j >= Q.j0 && j < Q.j1 && k >= Q.k0 && k < Q.k1;
+  auto holds = [&](int q) {  // cell in band q (only bands that overlap band r can hold it)
+    const Region& Q = bands.r[q];
+    return ((bands.ovl[r] >> q) & 1u) && i >= Q.i0 && i < Q.i1 && j >= Q.j0 && j < Q.j1 && k >= Q.k0 &&
+           k < Q.k1;
   };
   for (int r2 = 0; r2 < r; r2++)
-    if (holds(bands.r[r2])) return;  // owned by an earlier band
+    if (holds(r2)) return;  // owned by an earlier band
   const long long q = i * g.sx + j * g.sy + k;
   const int vpos[3] = {i, j, k};
   static constexpr int off[6][3] = {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   float dc[3] = {0.f, 0.f, 0.f};  // curl correction per component, all axes
   float dc2[3] = {0.f, 0.f, 0.f}; // its part on the stored split-field part (second curl axis)
   for (int rb = r; rb < bands.n; rb++) {
-    if (rb != r && !holds(bands.r[rb])) continue;
+    if (rb != r && !holds(rb)) continue;
     const int a = bands.axis[rb];
     // components with a derivative along a: c1 = (a+2)%3 (first curl term),
     // c2 = (a+1)%3 (second curl term); psi slots 1 and 0 of axis a.
@@ -1321,6 +1331,12 @@ static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const C
     const Region& R = b.r[r];
     n += (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
     b.start[r + 1] = (unsigned)std::min<long long>(n, 0xffffffffLL);
+    b.ovl[r] = 0;
+    for (int q = 0; q < b.n; q++) {
+      const Region& Q = b.r[q];
+      if (q != r && R.i0 < Q.i1 && Q.i0 < R.i1 && R.j0 < Q.j1 && Q.j0 < R.j1 && R.k0 < Q.k1 && Q.k0 < R.k1)
+        b.ovl[r] |= 1u << q;
+    }
   }
   if (n <= 0) return;
   if (n >= (1LL << 31)) {

@@ -525,13 +525,14 @@ static int setup_cpml(dtm_solver* s) {
   return 0;
 }
 
-// Launch layout.  The fused step and the one-cell-per-lane sweeps run the
-// whole grid in one launch of the variant that applies every boundary rule
-// (its x, y and z CPML branches are plane-, row- and column-uniform): at
-// 512x256x256 one launch beat an interior launch plus a y-strip launch by
-// 7-9%, because the strips' few tile columns forced short x segments.  The
-// paired FP16 sweeps keep the split: their variant with the y CPML would
-// spill, so the y-CPML rows run one cell per lane in a second launch.
+// Launch layout.  The fused step and the sweeps run the whole grid in one
+// launch of the variant that applies every boundary rule (its x, y and z
+// CPML branches are plane-, row- and column-uniform): at 512x256x256 one
+// launch beat an interior launch plus a y-strip launch by 7-9% (FP32) and
+// 4% (FP16 H), because the strips' few tile columns forced short x
+// segments.  The paired FP16 E sweep keeps the split: its variant with the y
+// CPML would spill, so the y-CPML rows run one cell per lane in a second
+// launch.
 static void setup_regions(dtm_solver* s) {
   const dtm_config& c = s->cfg;
   s->whole = Region{0, c.nx, 0, c.ny, 0, c.nz};
@@ -991,18 +992,9 @@ static int one_step(dtm_solver* s) {
       launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
   } else {
     // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
-    if (sweep_pairs(s->store, s->g, s->pm, nullptr)) {
-      ev_begin(s, 0);
-      launch_update_h(s->store, s->g, F, s->sc, s->pm, nullptr, 0, s->main2, s->stream);
-      ev_end(s);
-      ev_begin(s, 1);
-      launch_update_h(s->store, s->g, F, s->sc, s->pm, s->edge2, s->nedge2, kEmpty, s->stream);
-      ev_end(s);
-    } else {
-      ev_begin(s, 0);
-      launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, kEmpty, s->stream);
-      ev_end(s);
-    }
+    ev_begin(s, 0);
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, kEmpty, s->stream);
+    ev_end(s);
     if (s->tf.active) {
       ev_begin(s, 4);
       launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 0), s->bands, th, s->stream);
@@ -1328,7 +1320,7 @@ long long dtm_group_cells(const dtm_solver* s, int g) {
   };
   if (g < 0 || g > 3) return 0;
   if (s->torre) return g == 0 ? cells(s->whole) : 0;
-  const bool paired = sweep_pairs(s->store, s->g, s->pm, g < 2 ? nullptr : &s->mt);
+  const bool paired = g >= 2 && sweep_pairs(s->store, s->g, s->pm, &s->mt);  // the H sweep is one launch
   if (!paired) return (g == 0 || g == 2) ? cells(s->whole) : 0;
   if (g == 0 || g == 2) return cells(s->main2);
   long long t = 0;

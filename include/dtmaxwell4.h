@@ -257,9 +257,9 @@ int dtm_time_steps(dtm_solver* s, int nsteps, double* ms);
 const char* dtm_kernel_name(const dtm_solver* s);
 /* Per-launch HIP-event timing of the sweep kernels, recorded on the solver's
  * stream while enabled.  Groups: 0 H sweep (fused schedule: the whole step),
- * 1 H sweep of the y-CPML rows, 2 E sweep, 3 E sweep of the y-CPML rows
- * (groups 1 and 3 only with two FP16 cells per lane, whose groups 0 and 2
- * then hold the other rows), 4 TF/SF, ADE, sources and probes.
+ * 1 unused, 2 E sweep, 3 E sweep of the y-CPML rows (only with two FP16
+ * cells per lane, whose group 2 then holds the other rows), 4 TF/SF, ADE,
+ * sources and probes.
  * dtm_kernel_times returns the summed ms and launch counts per group and
  * resets them. */
 int dtm_set_kernel_timing(dtm_solver* s, int on);
