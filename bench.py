@@ -116,13 +116,15 @@ def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
     return res
 
 
-def adjoint_run(steps):
+def adjoint_run(steps, every=200):
     """Adjoint/forward time ratio on the inverse-design workload: 400x200x200,
     12-cell CPML, 64x64x32 design box (eps = 1 + 11 rho, rho ~ U[0,1] seeded,
     3-cell box smoothed), soft Ez dipole, J = sum |DFT Ez|^2 over a 16x16
     monitor plane at one frequency.  Forward = plain stepping of the same
     steps; adjoint = dtm_adjoint_design_gradient (its recorded forward pass
-    plus the backward pass), both wall-clock with the device synchronised."""
+    plus the backward pass), both wall-clock with the device synchronised;
+    timed with the design-box history and with whole-state checkpoints every
+    `every` steps plus recompute (the two gradients must be equal)."""
     import numpy as np
 
     from dtmaxwell4_amd import configs
@@ -148,14 +150,23 @@ def adjoint_run(steps):
     t = time.perf_counter()
     J, grad = s.design_gradient(steps, omega, mons)
     adj = time.perf_counter() - t
+    # the same gradient with whole-state checkpoints every `every` steps and recompute
+    s.set_adjoint_checkpoints(every)
+    t = time.perf_counter()
+    Jc, gradc = s.design_gradient(steps, omega, mons)
+    adjc = time.perf_counter() - t
     s.close()
     if not (np.isfinite(J) and np.isfinite(grad).all()):
         raise RuntimeError("non-finite adjoint result")
+    if Jc != J or not np.array_equal(gradc, grad):
+        raise RuntimeError("checkpointed adjoint differs from the history-mode adjoint")
     cells = w.cfg.nx * w.cfg.ny * w.cfg.nz
     return dict(ratio=adj / fwd, forward_s=fwd, adjoint_call_s=adj, steps=steps,
+                checkpoint_every=every, ratio_checkpointed=adjc / fwd, adjoint_call_checkpointed_s=adjc,
                 forward_gcells=cells * steps / fwd / 1e9, J=J, grad_norm=float(np.linalg.norm(grad)),
                 workload=f"inverse design {w.cfg.nx}x{w.cfg.ny}x{w.cfg.nz}, design {list(eps.shape)}, "
-                         f"{len(mons)} monitors, {steps} steps (call = recorded forward + backward)")
+                         f"{len(mons)} monitors, {steps} steps (call = recorded forward + backward; "
+                         f"checkpointed: state every {every} steps + recompute)")
 
 
 def purcell_run(steps):
@@ -254,7 +265,8 @@ def main():
     if not args.no_adjoint:
         adj = adjoint_run(args.adjoint_steps)
         log(f"[bench] adjoint/forward {adj['ratio']:.3f} (forward {adj['forward_s']:.3f} s, "
-            f"adjoint call {adj['adjoint_call_s']:.3f} s)")
+            f"adjoint call {adj['adjoint_call_s']:.3f} s); checkpointed every {adj['checkpoint_every']}: "
+            f"{adj['ratio_checkpointed']:.3f}")
     pur = None
     if not args.no_purcell:
         pur = purcell_run(args.purcell_steps)

@@ -164,6 +164,7 @@ def load_library():
         "dtm_field_write": (C.c_int, [P, C.c_int, P]),
         "dtm_energy": (C.c_int, [P, C.POINTER(C.c_double)]),
         "dtm_adjoint_design_gradient": (C.c_int, [P, C.c_int, C.c_double, C.c_int, P, C.POINTER(C.c_double), P]),
+        "dtm_adjoint_set_checkpoints": (C.c_int, [P, C.c_int]),
         "dtm_time_steps": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
         "dtm_kernel_name": (C.c_char_p, [P]),
         "dtm_set_kernel_timing": (C.c_int, [P, C.c_int]),

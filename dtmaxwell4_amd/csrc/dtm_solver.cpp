@@ -64,6 +64,8 @@ struct dtm_solver {
   std::vector<void*> allocs;  // everything to free
   std::vector<std::pair<void*, size_t>> state;  // field / psi buffers (zeroed by a reset)
   bool adjoint = false;  // stepping adjoint fields: no sources, no recording
+  bool replay = false;   // recomputing a forward segment of the adjoint: no recording
+  int adj_every = 0;     // adjoint: 0 = keep the design-box history, K > 0 = checkpoint every K steps
   Region whole{};      // the grid: one launch of the variant with every boundary rule
   Region main2{};      // paired two-pass sweeps: all x, y outside the y CPML
   Region edge2[2]{};   // ... and the y-CPML rows
@@ -1039,7 +1041,7 @@ static int one_step(dtm_solver* s) {
     }
     launch_soft(s->store, s->g, F, s->sc, so, s->stream);
   }
-  const int np = s->adjoint ? 0 : (int)s->probes.size();
+  const int np = (s->adjoint || s->replay) ? 0 : (int)s->probes.size();
   if (np > 0) {
     Probes pr{};
     pr.n = np;
@@ -1109,6 +1111,59 @@ int dtm_dft_read(dtm_solver* s, double* out) {
 
 static void* comp_ptr(dtm_solver* s, int comp) { return comp < 3 ? s->E[comp] : s->H[comp - 3]; }
 
+// Time-stepped state of the solver: every registered buffer's contents plus
+// which of the double buffers is current.
+struct StateImage {
+  void *E[3], *H[3], *E2[3], *H2[3];
+  Cpml pm, pmn;
+};
+static StateImage state_ptrs(const dtm_solver* s) {
+  StateImage im;
+  for (int c = 0; c < 3; c++) {
+    im.E[c] = s->E[c];
+    im.H[c] = s->H[c];
+    im.E2[c] = s->E2[c];
+    im.H2[c] = s->H2[c];
+  }
+  im.pm = s->pm;
+  im.pmn = s->pmn;
+  return im;
+}
+static size_t state_bytes(const dtm_solver* s) {
+  size_t t = 0;
+  for (auto& b : s->state) t += b.second;
+  return t;
+}
+static int save_state(dtm_solver* s, char* dst) {
+  for (auto& b : s->state) {
+    HIPCHK(hipMemcpyAsync(dst, b.first, b.second, hipMemcpyDeviceToDevice, s->stream));
+    dst += b.second;
+  }
+  return 0;
+}
+static int load_state(dtm_solver* s, const char* src, const StateImage& im) {
+  for (auto& b : s->state) {
+    HIPCHK(hipMemcpyAsync(b.first, src, b.second, hipMemcpyDeviceToDevice, s->stream));
+    src += b.second;
+  }
+  for (int c = 0; c < 3; c++) {
+    s->E[c] = im.E[c];
+    s->H[c] = im.H[c];
+    s->E2[c] = im.E2[c];
+    s->H2[c] = im.H2[c];
+  }
+  s->pm = im.pm;
+  s->pmn = im.pmn;
+  field_structs(s);
+  return 0;
+}
+
+int dtm_adjoint_set_checkpoints(dtm_solver* s, int every) {
+  if (!s || every < 0) return fail(DTM_ERR_ARG, "checkpoint interval must be >= 0");
+  s->adj_every = every;
+  return 0;
+}
+
 int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmon, const int* mon_ijk,
                                 double* J_out, double* grad_inv_eps) {
   if (!s || nsteps < 1 || nmon < 1 || !mon_ijk || !J_out || !grad_inv_eps)
@@ -1132,21 +1187,30 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk)
       ie[m] = s->design_host[((size_t)di * mt.d_nj + dj) * mt.d_nk + dk];
   }
-  // temporaries: monitor cells, the Ez series, the design-box E history
-  // (nsteps+1 snapshots), the gradient accumulator
-  const size_t hist_bytes = sizeof(float) * 3 * (size_t)nd * ((size_t)nsteps + 1);
+  // History mode keeps the design-box E of every step (nsteps+1 snapshots).
+  // Checkpoint mode (interval K) keeps the whole state every K steps and,
+  // segment by segment on the way back, recomputes the segment forward to
+  // rebuild its K+1 snapshots; the adjoint state waits in a stash meanwhile.
+  const bool ckpt = s->adj_every > 0 && s->adj_every < nsteps;
+  const int K = ckpt ? s->adj_every : nsteps;
+  const int nck = ckpt ? (nsteps + K - 1) / K : 0;
+  const size_t sbytes = ckpt ? state_bytes(s) : 0;
+  const size_t hist_bytes = sizeof(float) * 3 * (size_t)nd * ((size_t)K + 1);
   void *d_q = nullptr, *d_ie = nullptr, *d_ser = nullptr, *d_hist = nullptr, *d_acc = nullptr, *d_F = nullptr;
+  void *d_ck = nullptr, *d_stash = nullptr;
   auto release = [&]() {
-    for (void* p : {d_q, d_ie, d_ser, d_hist, d_acc, d_F})
+    for (void* p : {d_q, d_ie, d_ser, d_hist, d_acc, d_F, d_ck, d_stash})
       if (p) (void)hipFree(p);
   };
   if (hipMalloc(&d_q, sizeof(long long) * nmon) != hipSuccess || hipMalloc(&d_ie, sizeof(float) * nmon) != hipSuccess ||
       hipMalloc(&d_ser, sizeof(float) * (size_t)nmon * nsteps) != hipSuccess ||
       hipMalloc(&d_hist, hist_bytes) != hipSuccess || hipMalloc(&d_acc, sizeof(double) * nd) != hipSuccess ||
-      hipMalloc(&d_F, sizeof(double) * 2 * nmon) != hipSuccess) {
+      hipMalloc(&d_F, sizeof(double) * 2 * nmon) != hipSuccess ||
+      (ckpt && (hipMalloc(&d_ck, sbytes * nck) != hipSuccess || hipMalloc(&d_stash, sbytes) != hipSuccess))) {
     (void)hipGetLastError();
     release();
-    return fail(DTM_ERR_NOMEM, "adjoint temporaries (design-box history " + std::to_string(hist_bytes >> 20) + " MiB)");
+    return fail(DTM_ERR_NOMEM, "adjoint temporaries (design-box history " + std::to_string(hist_bytes >> 20) +
+                                   " MiB, checkpoints " + std::to_string((sbytes * (nck + 1)) >> 20) + " MiB)");
   }
   auto* hq = static_cast<long long*>(d_q);
   auto* hie = static_cast<float*>(d_ie);
@@ -1154,6 +1218,9 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
   auto* hist = static_cast<float*>(d_hist);
   auto* acc = static_cast<double*>(d_acc);
   auto* Fd = static_cast<double*>(d_F);
+  auto* ck = static_cast<char*>(d_ck);
+  auto* stash = static_cast<char*>(d_stash);
+  std::vector<StateImage> ck_ptrs(nck);
   const size_t snap = 3 * (size_t)nd;
   int rc = 0;
   do {
@@ -1163,12 +1230,17 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
       rc = fail(DTM_ERR_DEVICE, "adjoint set-up copies");
       break;
     }
-    // forward: record Ez at the monitors and the design-box E every step
-    launch_adj_snap(s->g, fields_ptr(s), mt, hist, s->stream);
+    // forward: record Ez at the monitors every step, and the design-box E
+    // every step (history) or the whole state every K steps (checkpoints)
+    if (!ckpt) launch_adj_snap(s->g, fields_ptr(s), mt, hist, s->stream);
     for (int n = 0; n < nsteps && !rc; n++) {
+      if (ckpt && n % K == 0) {
+        ck_ptrs[n / K] = state_ptrs(s);
+        if ((rc = save_state(s, ck + (size_t)(n / K) * sbytes))) break;
+      }
       rc = one_step(s);
       launch_adj_monitor(fields_ptr(s), hq, nmon, ser + (size_t)n * nmon, s->stream);
-      launch_adj_snap(s->g, fields_ptr(s), mt, hist + (size_t)(n + 1) * snap, s->stream);
+      if (!ckpt) launch_adj_snap(s->g, fields_ptr(s), mt, hist + (size_t)(n + 1) * snap, s->stream);
     }
     if (rc) break;
     std::vector<float> hs((size_t)nmon * nsteps);
@@ -1199,12 +1271,35 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     if ((rc = zero_state(s))) break;
     s->adjoint = true;
     launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, nsteps, s->stream);
-    for (int n = nsteps - 1; n >= 0 && !rc; n--) {
-      launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)n * snap, hist + (size_t)(n + 1) * snap, acc, s->stream);
-      rc = one_step(s);
-      if (n >= 1) launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
+    for (int seg = (ckpt ? nck - 1 : 0); seg >= 0 && !rc; seg--) {
+      const int n0 = seg * K, n1 = std::min(n0 + K, nsteps);
+      if (ckpt) {  // rebuild the segment's history from its checkpoint, the adjoint state stashed
+        const StateImage adj = state_ptrs(s);
+        const long long adj_step = s->step;
+        if ((rc = save_state(s, stash)) || (rc = load_state(s, ck + (size_t)seg * sbytes, ck_ptrs[seg]))) break;
+        s->adjoint = false;
+        s->replay = true;
+        s->step = n0;
+        s->tab_n0 = -1;
+        launch_adj_snap(s->g, fields_ptr(s), mt, hist, s->stream);
+        for (int n = n0; n < n1 && !rc; n++) {
+          rc = one_step(s);
+          launch_adj_snap(s->g, fields_ptr(s), mt, hist + (size_t)(n - n0 + 1) * snap, s->stream);
+        }
+        s->replay = false;
+        s->adjoint = true;
+        if (rc || (rc = load_state(s, stash, adj))) break;
+        s->step = adj_step;
+      }
+      for (int n = n1 - 1; n >= n0 && !rc; n--) {
+        launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n - n0) * snap, hist + (size_t)(n - n0 + 1) * snap,
+                        acc, s->stream);
+        rc = one_step(s);
+        if (n >= 1) launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
+      }
     }
     s->adjoint = false;
+    s->replay = false;
     if (rc) break;
     if (hipMemcpyAsync(grad_inv_eps, acc, sizeof(double) * nd, hipMemcpyDeviceToHost, s->stream) != hipSuccess) {
       rc = fail(DTM_ERR_DEVICE, "adjoint gradient copy");
@@ -1212,6 +1307,7 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     }
   } while (false);
   s->adjoint = false;
+  s->replay = false;
   // back to step 0 with zero fields
   if (!rc) rc = zero_state(s);
   s->step = 0;

@@ -147,6 +147,11 @@ class Solver:
         abi.check(self.L.dtm_energy(self.h, C.byref(v)), "dtm_energy")
         return v.value
 
+    def set_adjoint_checkpoints(self, every):
+        """Adjoint memory scheme: 0 keeps the design-box E of every step;
+        K > 0 keeps the whole state every K steps and recomputes each segment."""
+        abi.check(self.L.dtm_adjoint_set_checkpoints(self.h, int(every)), "dtm_adjoint_set_checkpoints")
+
     def design_gradient(self, nsteps, omega, monitors):
         """Adjoint gradient of J = sum_m |DFT_omega Ez_m|^2 over nsteps with
         respect to the design box's 1/eps.  monitors: (i, j, k) cells.

@@ -249,6 +249,14 @@ int dtm_energy(dtm_solver* s, double* out);
  * (it has no adjoint); the explicit API's dtm_adjoint entry. */
 int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmon, const int* mon_ijk,
                                 double* J_out, double* grad_inv_eps);
+/* Memory scheme of dtm_adjoint_design_gradient.  every = 0 (default) keeps
+ * the design-box E of every step.  every = K > 0 keeps the whole time-stepped
+ * state every K steps instead and, segment by segment on the way back,
+ * recomputes each K steps forward to rebuild their design-box E: device memory
+ * (ceil(nsteps/K) + 1) states plus K+1 design-box snapshots, for one more
+ * forward pass.  For grids whose design box is a large part of the domain.
+ * No counterpart in the reference (it has no adjoint). */
+int dtm_adjoint_set_checkpoints(dtm_solver* s, int every);
 
 /* ---------------------------------------------------------------- timing */
 /* Time `nsteps` steps with HIP events on the solver's stream; returns ms. */
