@@ -78,11 +78,16 @@ def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
         f"sweep={'fused' if sweep else 'twopass'} warmup={warmup} steps={steps}")
     s.step(warmup)
     s.sync()
-    s.set_kernel_timing(True)
-    s.kernel_times()  # reset
+    # throughput: HIP events around the K steps only (per-launch events would
+    # add a gap after every kernel)
     t0 = time.perf_counter()
     ms = s.time_steps(steps)
     wall = time.perf_counter() - t0
+    # per-kernel times for the roofline: a second pass with HIP events around
+    # every launch group, on the solver's stream
+    s.set_kernel_timing(True)
+    s.kernel_times()  # reset
+    s.time_steps(min(steps, 100))
     gms, gcnt = s.kernel_times()
     s.set_kernel_timing(False)
     cells = nx * ny * nz
