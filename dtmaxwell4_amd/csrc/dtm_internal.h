@@ -199,18 +199,12 @@ struct Soft {
   float val[16];  // dt * inv_eps * J(t)
 };
 
-// Launchers (dtm_kernels.hip).  All enqueue on `st`.  `main` runs the
-// interior variant (no y CPML); `regs` run the variant with every boundary
-// rule.
-// True when a two-pass sweep runs two z cells per lane (FP16 storage, even
-// nz; for the E sweep (mt given) also no design box; never with the split
-// layer).  The paired interior variant then needs the y-CPML rows apart.
-bool sweep_pairs(int store, const Grid& g, const Cpml& pm, const Materials* mt);
+// Launchers (dtm_kernels.hip).  All enqueue on `st`; each sweep is one
+// launch of the kernel variant with every boundary rule over `regs`.
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Region* regs, int nreg, const Region& main, hipStream_t st);
+                     const Region* regs, int nreg, hipStream_t st);
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, const Region& main,
-                     hipStream_t st);
+                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st);
 // TF/SF corrections applied after a sweep (half = 0: H, 1: E).  The update is
 // linear in the stencil sums, so the correction is additive, including its
 // CPML psi and ADE J terms.
@@ -234,7 +228,7 @@ void launch_ade(int store, const Grid& g, const void* fields_src, const void* fi
 // Fused time step (dtm_torre.hip): old fields/psi -> new fields/psi.
 // `cold` is a device copy of the step's TorreCold (make_torre_cold).
 void launch_torre(int store, const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
-                  const Region* regs, int nreg, const Region& main, hipStream_t st);
+                  const Region* regs, int nreg, hipStream_t st);
 TorreCold make_torre_cold(const void* fields_old, const void* fields_new, int store, const Cpml& pm_old,
                           const Cpml& pm_new, const Materials& mt, const Ade& ade);
 void launch_soft(int store, const Grid& g, const void* fields, const Scales& sc, const Soft& so,

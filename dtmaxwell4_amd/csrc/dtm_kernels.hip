@@ -720,6 +720,11 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
     if (holds(r2)) return;  // owned by an earlier band
   const long long q = i * g.sx + j * g.sy + k;
   const int vpos[3] = {i, j, k};
+  // the values to update, loaded before the corrections are evaluated so
+  // their latency overlaps that work (most band cells update two of them)
+  float old[3];
+  for (int c = 0; c < 3; c++) old[c] = HALF == 0 ? ld(f.H[c], q, sc.ih[c]) : ld(f.E[c], q, sc.ie[c]);
+  const uint8_t cls = (HALF == 1 && mt.cls) ? mt.cls[q] : 0;
   static constexpr int off[6][3] = {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   float dc[3] = {0.f, 0.f, 0.f};  // curl correction per component, all axes
   float dc2[3] = {0.f, 0.f, 0.f}; // its part on the stored split-field part (second curl axis)
@@ -765,12 +770,12 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
   }
   if (HALF == 0) {
     for (int c = 0; c < 3; c++) {
-      if (dc[c] != 0.0f) st(f.H[c], q, ld(f.H[c], q, sc.ih[c]) - g.dt * dc[c], sc.sh[c]);
+      if (dc[c] != 0.0f) st(f.H[c], q, old[c] - g.dt * dc[c], sc.sh[c]);
       if (dc2[c] != 0.0f) pm.p2H[c][q] -= g.dt * dc2[c];
     }
     return;
   }
-  const ClassEntry ce = mt.table[mt.cls ? mt.cls[q] : 0];
+  const ClassEntry ce = mt.table[cls];
   bool over = false;
   float dv = 0.f;
   if (mt.d_on) {
@@ -786,11 +791,11 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
       const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
       const AdeMat& am = ade.mats[ce.mat[c]];
       const float dE = am.kE[2] * g.dt * dc[c];
-      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + dE, sc.se[c]);
+      st(f.E[c], q, old[c] + dE, sc.se[c]);
       for (int p = 0; p < am.np; p++) ade.Jp[c][p][b] += am.kJ[p][2] * dE;
     } else {
       const float ie = over ? dv : ce.inv_eps[c];
-      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + g.dt * ie * dc[c], sc.se[c]);
+      st(f.E[c], q, old[c] + g.dt * ie * dc[c], sc.se[c]);
       if (dc2[c] != 0.0f) pm.p2E[c][q] += g.dt * ie * dc2[c];
     }
   }
@@ -1239,42 +1244,27 @@ static int march_seg(const Region* regs, int nreg, int slots, int bz) {
 
 template <int STORE, int V, bool SP>
 static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
-                  const Region& main, hipStream_t st) {
+                  hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, MBY, 1);
-  static const int slots_m = resident_blocks(k_update_h<STORE, 1, V, SP>);
-  static const int slots_e = resident_blocks(k_update_h<STORE, 3, V, SP>);
-  const int nm = main.i1 > main.i0 ? 1 : 0;
-  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m, BZ * V), BZ * V);
-  if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, 1, V, SP>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, rm);
-    check_launch("k_update_h<main>");
-  }
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e, BZ * V), BZ * V);
+  static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP>);
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V), BZ * V);
   if (rg.total > 0) {
     hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
-    check_launch("k_update_h<edge>");
+    check_launch("k_update_h");
   }
 }
 
 template <int STORE, int V, bool SP>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
-                  const Ade& ade, const Region* regs, int nreg, const Region& main, hipStream_t st) {
+                  const Ade& ade, const Region* regs, int nreg, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   const dim3 blk(BZ, MBY, 1);
-  // the y-CPML edge keeps one cell per lane (the paired variant would spill)
-  static const int slots_m = resident_blocks(k_update_e<STORE, 1, V, SP>);
-  static const int slots_e = resident_blocks(k_update_e<STORE, 3, 1, SP>);
-  const int nm = main.i1 > main.i0 ? 1 : 0;
-  MarchSet rm = make_march(&main, nm, march_seg(&main, nm, slots_m, BZ * V), BZ * V);
-  if (rm.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, 1, V, SP>), dim3(rm.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rm);
-    check_launch("k_update_e<main>");
-  }
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots_e, BZ), BZ);
+  static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP>);
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V), BZ * V);
   if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, 3, 1, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
-    check_launch("k_update_e<edge>");
+    hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
+    check_launch("k_update_e");
   }
 }
 
@@ -1284,36 +1274,31 @@ static bool pair_cells(int store, const Grid& g, const Materials* mt) {
   return store == 1 && g.nz % 2 == 0 && !(mt && mt->d_on);
 }
 
-bool sweep_pairs(int store, const Grid& g, const Cpml& pm, const Materials* mt) {
-  return !pm.split && pair_cells(store, g, mt);
-}
-
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Region* regs, int nreg, const Region& main, hipStream_t st) {
+                     const Region* regs, int nreg, hipStream_t st) {
   if (pm.split) {  // split-field layer: one cell per lane
-    if (store == 0) upd_h<0, 1, true>(g, fields, sc, pm, regs, nreg, main, st);
-    else upd_h<1, 1, true>(g, fields, sc, pm, regs, nreg, main, st);
+    if (store == 0) upd_h<0, 1, true>(g, fields, sc, pm, regs, nreg, st);
+    else upd_h<1, 1, true>(g, fields, sc, pm, regs, nreg, st);
   } else if (store == 0) {
-    upd_h<0, 1, false>(g, fields, sc, pm, regs, nreg, main, st);
+    upd_h<0, 1, false>(g, fields, sc, pm, regs, nreg, st);
   } else if (pair_cells(store, g, nullptr)) {
-    upd_h<1, 2, false>(g, fields, sc, pm, regs, nreg, main, st);
+    upd_h<1, 2, false>(g, fields, sc, pm, regs, nreg, st);
   } else {
-    upd_h<1, 1, false>(g, fields, sc, pm, regs, nreg, main, st);
+    upd_h<1, 1, false>(g, fields, sc, pm, regs, nreg, st);
   }
 }
 
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, const Region& main,
-                     hipStream_t st) {
+                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st) {
   if (pm.split) {
-    if (store == 0) upd_e<0, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
-    else upd_e<1, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+    if (store == 0) upd_e<0, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, st);
+    else upd_e<1, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, st);
   } else if (store == 0) {
-    upd_e<0, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+    upd_e<0, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, st);
   } else if (pair_cells(store, g, &mt)) {
-    upd_e<1, 2, false>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+    upd_e<1, 2, false>(g, fields, sc, pm, mt, ade, regs, nreg, st);
   } else {
-    upd_e<1, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, main, st);
+    upd_e<1, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, st);
   }
 }
 

@@ -67,9 +67,6 @@ struct dtm_solver {
   bool replay = false;   // recomputing a forward segment of the adjoint: no recording
   int adj_every = 0;     // adjoint: 0 = keep the design-box history, K > 0 = checkpoint every K steps
   Region whole{};      // the grid: one launch of the variant with every boundary rule
-  Region main2{};      // paired two-pass sweeps: all x, y outside the y CPML
-  Region edge2[2]{};   // ... and the y-CPML rows
-  int nedge2 = 0;
   // materials
   std::vector<uint8_t> cls_host;
   std::vector<ClassEntry> classes;
@@ -530,19 +527,12 @@ static int setup_cpml(dtm_solver* s) {
 // Launch layout.  The fused step and the sweeps run the whole grid in one
 // launch of the variant that applies every boundary rule (its x, y and z
 // CPML branches are plane-, row- and column-uniform): at 512x256x256 one
-// launch beat an interior launch plus a y-strip launch by 7-9% (FP32) and
-// 4% (FP16 H), because the strips' few tile columns forced short x
-// segments.  The paired FP16 E sweep keeps the split: its variant with the y
-// CPML would spill, so the y-CPML rows run one cell per lane in a second
-// launch.
+// launch beat an interior launch plus a y-strip launch by 7-9% (FP32),
+// 4% (FP16 H) and 4.5% (FP16 E), because the strips' few tile columns
+// forced short x segments.
 static void setup_regions(dtm_solver* s) {
   const dtm_config& c = s->cfg;
   s->whole = Region{0, c.nx, 0, c.ny, 0, c.nz};
-  const int yl = std::min(c.npml[2], c.ny), yh = std::max(c.ny - c.npml[3], yl);
-  s->main2 = Region{0, c.nx, yl, yh, 0, c.nz};
-  s->nedge2 = 0;
-  if (yl > 0) s->edge2[s->nedge2++] = Region{0, c.nx, 0, yl, 0, c.nz};
-  if (yh < c.ny) s->edge2[s->nedge2++] = Region{0, c.nx, yh, c.ny, 0, c.nz};
 }
 
 static void field_structs(dtm_solver* s) {
@@ -924,7 +914,6 @@ static int flush_series(dtm_solver* s) {
   return 0;
 }
 
-static const Region kEmpty{0, 0, 0, 0, 0, 0};
 
 static void ev_begin(dtm_solver* s, int group) {
   if (!s->timing) return;
@@ -984,7 +973,7 @@ static int one_step(dtm_solver* s) {
     const TorreCold* cold = torre_cold(s);
     if (!cold) return DTM_ERR_DEVICE;
     ev_begin(s, 0);
-    launch_torre(s->store, s->g, s->sc, s->mt, cold, &s->whole, 1, kEmpty, s->stream);
+    launch_torre(s->store, s->g, s->sc, s->mt, cold, &s->whole, 1, s->stream);
     ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, fields_next_ptr(s), s->sc, s->mt, s->ade, s->stream);
@@ -995,7 +984,7 @@ static int one_step(dtm_solver* s) {
   } else {
     // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
     ev_begin(s, 0);
-    launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, kEmpty, s->stream);
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, s->stream);
     ev_end(s);
     if (s->tf.active) {
       ev_begin(s, 4);
@@ -1003,18 +992,9 @@ static int one_step(dtm_solver* s) {
       ev_end(s);
     }
     // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
-    if (sweep_pairs(s->store, s->g, s->pm, &s->mt)) {
-      ev_begin(s, 2);
-      launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, nullptr, 0, s->main2, s->stream);
-      ev_end(s);
-      ev_begin(s, 3);
-      launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, s->edge2, s->nedge2, kEmpty, s->stream);
-      ev_end(s);
-    } else {
-      ev_begin(s, 2);
-      launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, &s->whole, 1, kEmpty, s->stream);
-      ev_end(s);
-    }
+    ev_begin(s, 2);
+    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, &s->whole, 1, s->stream);
+    ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, F, s->sc, s->mt, s->ade, s->stream);
     if (s->tf.active)
@@ -1416,12 +1396,7 @@ long long dtm_group_cells(const dtm_solver* s, int g) {
   };
   if (g < 0 || g > 3) return 0;
   if (s->torre) return g == 0 ? cells(s->whole) : 0;
-  const bool paired = g >= 2 && sweep_pairs(s->store, s->g, s->pm, &s->mt);  // the H sweep is one launch
-  if (!paired) return (g == 0 || g == 2) ? cells(s->whole) : 0;
-  if (g == 0 || g == 2) return cells(s->main2);
-  long long t = 0;
-  for (int q = 0; q < s->nedge2; q++) t += cells(s->edge2[q]);
-  return t;
+  return (g == 0 || g == 2) ? cells(s->whole) : 0;
 }
 
 }  // extern "C"

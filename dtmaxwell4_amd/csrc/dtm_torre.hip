@@ -554,20 +554,13 @@ static int torre_slots(K kernel) {
 
 template <int STORE>
 static void torre_go(const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
-                     const Region* regs, int nreg, const Region& main, hipStream_t st) {
+                     const Region* regs, int nreg, hipStream_t st) {
   const dim3 blk(TZ, TR, 1);
-  static const int slots_m = torre_slots(k_torre<STORE, 1>);
-  static const int slots_e = torre_slots(k_torre<STORE, 3>);
-  const int nm = main.i1 > main.i0 ? 1 : 0;
-  TorreSet tm = make_torre_set(&main, nm, torre_seg(&main, nm, slots_m));
-  if (tm.total > 0) {
-    hipLaunchKernelGGL((k_torre<STORE, 1>), dim3(tm.per * NXCD), blk, 0, st, g, sc, mt.cls, cold, tm);
-    check_launch("k_torre<main>");
-  }
-  TorreSet te = make_torre_set(regs, nreg, torre_seg(regs, nreg, slots_e));
+  static const int slots = torre_slots(k_torre<STORE, 3>);
+  TorreSet te = make_torre_set(regs, nreg, torre_seg(regs, nreg, slots));
   if (te.total > 0) {
     hipLaunchKernelGGL((k_torre<STORE, 3>), dim3(te.per * NXCD), blk, 0, st, g, sc, mt.cls, cold, te);
-    check_launch("k_torre<edge>");
+    check_launch("k_torre");
   }
 }
 
@@ -608,9 +601,9 @@ TorreCold make_torre_cold(const void* fields_old, const void* fields_new, int st
 }
 
 void launch_torre(int store, const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
-                  const Region* regs, int nreg, const Region& main, hipStream_t st) {
-  if (store == 0) torre_go<0>(g, sc, mt, cold, regs, nreg, main, st);
-  else torre_go<1>(g, sc, mt, cold, regs, nreg, main, st);
+                  const Region* regs, int nreg, hipStream_t st) {
+  if (store == 0) torre_go<0>(g, sc, mt, cold, regs, nreg, st);
+  else torre_go<1>(g, sc, mt, cold, regs, nreg, st);
 }
 
 }  // namespace dtm
