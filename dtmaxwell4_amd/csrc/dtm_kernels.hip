@@ -743,9 +743,15 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
       if (D == 0.0f) continue;
       float cp = 0.0f;
       if (pm.on && pm.split) {  // split-field layer: the part of axis a gets k2 * D
-        const int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
-        if (slab_index(vpos[a], n, pm.plo[a], pm.phi[a]) >= 0) {
-          const float k2 = HALF == 0 ? pm.cH[a][vpos[a]] : pm.cE[a][vpos[a]];
+        auto k2_of = [&](int ax) {  // 1 off the slab of axis ax
+          const int n = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+          if (slab_index(vpos[ax], n, pm.plo[ax], pm.phi[ax]) < 0) return 1.0f;
+          return HALF == 0 ? pm.cH[ax][vpos[ax]] : pm.cE[ax][vpos[ax]];
+        };
+        const float k2 = k2_of(a);
+        // the cell keeps split parts when either curl axis has k2 != 1; the
+        // second curl axis (t = 1) owns the stored part
+        if (k2 != 1.0f || k2_of(t == 0 ? (c + 2) % 3 : (c + 1) % 3) != 1.0f) {
           dc[c] += sgn * k2 * D;
           if (t == 1) dc2[c] += sgn * k2 * D;
           continue;
@@ -1302,9 +1308,9 @@ void launch_update_e(int store, const Grid& g, const void* fields, const Scales&
   }
 }
 
-// One launch per face-normal axis: bands of different axes overlap at the
-// box edges and both update the shared component there (e.g. Hz from the x
-// and y faces), so they must not run concurrently.
+// One launch over the union of the face bands (k_tfsf gives each cell one
+// writer); the host fills each band's first thread index and its overlap
+// mask.
 template <int STORE, int MODE>
 static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                     const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double tinc, hipStream_t st) {
