@@ -1171,6 +1171,8 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
   // Checkpoint mode (interval K) keeps the whole state every K steps and,
   // segment by segment on the way back, recomputes the segment forward to
   // rebuild its K+1 snapshots; the adjoint state waits in a stash meanwhile.
+  // The last segment's snapshots are taken during the forward pass, so the
+  // way back starts without a recompute.
   const bool ckpt = s->adj_every > 0 && s->adj_every < nsteps;
   const int K = ckpt ? s->adj_every : nsteps;
   const int nck = ckpt ? (nsteps + K - 1) / K : 0;
@@ -1212,15 +1214,17 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     }
     // forward: record Ez at the monitors every step, and the design-box E
     // every step (history) or the whole state every K steps (checkpoints)
-    if (!ckpt) launch_adj_snap(s->g, fields_ptr(s), mt, hist, s->stream);
+    // plus the last segment's snapshots
+    const int n_last = ckpt ? (nck - 1) * K : 0;  // first step of the last segment
     for (int n = 0; n < nsteps && !rc; n++) {
       if (ckpt && n % K == 0) {
         ck_ptrs[n / K] = state_ptrs(s);
         if ((rc = save_state(s, ck + (size_t)(n / K) * sbytes))) break;
       }
+      if (n == n_last) launch_adj_snap(s->g, fields_ptr(s), mt, hist, s->stream);
       rc = one_step(s);
       launch_adj_monitor(fields_ptr(s), hq, nmon, ser + (size_t)n * nmon, s->stream);
-      if (!ckpt) launch_adj_snap(s->g, fields_ptr(s), mt, hist + (size_t)(n + 1) * snap, s->stream);
+      if (n >= n_last) launch_adj_snap(s->g, fields_ptr(s), mt, hist + (size_t)(n - n_last + 1) * snap, s->stream);
     }
     if (rc) break;
     std::vector<float> hs((size_t)nmon * nsteps);
@@ -1253,7 +1257,7 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, nsteps, s->stream);
     for (int seg = (ckpt ? nck - 1 : 0); seg >= 0 && !rc; seg--) {
       const int n0 = seg * K, n1 = std::min(n0 + K, nsteps);
-      if (ckpt) {  // rebuild the segment's history from its checkpoint, the adjoint state stashed
+      if (ckpt && seg < nck - 1) {  // rebuild the segment's history from its checkpoint, the adjoint state stashed
         const StateImage adj = state_ptrs(s);
         const long long adj_step = s->step;
         if ((rc = save_state(s, stash)) || (rc = load_state(s, ck + (size_t)seg * sbytes, ck_ptrs[seg]))) break;
