@@ -720,11 +720,6 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
     if (holds(r2)) return;  // owned by an earlier band
   const long long q = i * g.sx + j * g.sy + k;
   const int vpos[3] = {i, j, k};
-  // the values to update, loaded before the corrections are evaluated so
-  // their latency overlaps that work (most band cells update two of them)
-  float old[3];
-  for (int c = 0; c < 3; c++) old[c] = HALF == 0 ? ld(f.H[c], q, sc.ih[c]) : ld(f.E[c], q, sc.ie[c]);
-  const uint8_t cls = (HALF == 1 && mt.cls) ? mt.cls[q] : 0;
   static constexpr int off[6][3] = {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   float dc[3] = {0.f, 0.f, 0.f};  // curl correction per component, all axes
   float dc2[3] = {0.f, 0.f, 0.f}; // its part on the stored split-field part (second curl axis)
@@ -776,12 +771,12 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
   }
   if (HALF == 0) {
     for (int c = 0; c < 3; c++) {
-      if (dc[c] != 0.0f) st(f.H[c], q, old[c] - g.dt * dc[c], sc.sh[c]);
+      if (dc[c] != 0.0f) st(f.H[c], q, ld(f.H[c], q, sc.ih[c]) - g.dt * dc[c], sc.sh[c]);
       if (dc2[c] != 0.0f) pm.p2H[c][q] -= g.dt * dc2[c];
     }
     return;
   }
-  const ClassEntry ce = mt.table[cls];
+  const ClassEntry ce = mt.table[mt.cls ? mt.cls[q] : 0];
   bool over = false;
   float dv = 0.f;
   if (mt.d_on) {
@@ -797,11 +792,11 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
       const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
       const AdeMat& am = ade.mats[ce.mat[c]];
       const float dE = am.kE[2] * g.dt * dc[c];
-      st(f.E[c], q, old[c] + dE, sc.se[c]);
+      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + dE, sc.se[c]);
       for (int p = 0; p < am.np; p++) ade.Jp[c][p][b] += am.kJ[p][2] * dE;
     } else {
       const float ie = over ? dv : ce.inv_eps[c];
-      st(f.E[c], q, old[c] + g.dt * ie * dc[c], sc.se[c]);
+      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + g.dt * ie * dc[c], sc.se[c]);
       if (dc2[c] != 0.0f) pm.p2E[c][q] += g.dt * ie * dc2[c];
     }
   }
