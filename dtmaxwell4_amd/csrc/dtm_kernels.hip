@@ -238,13 +238,16 @@ __device__ __forceinline__ void cpml_step(float* psi, long long pq, float old, f
 }
 
 // ===================================================== marching sweeps
-// A block owns a tile of 64 (z) x MBY (y) columns and marches along x over
+// A block owns a tile of 64 (z) x tile_rows(V) (y) columns and marches along x over
 // a segment of planes.  Per thread, the x-neighbours of its column live in a
 // 4-deep register queue (the tilted space-time march of the reference's
 // torre, genAlg.py:420-499, with the rag shuffles of copyrags.cuh replaced
 // by a register queue); the y/z neighbours of the current plane come from a
 // double-buffered LDS tile with a 3-cell halo.  One barrier per plane.
-constexpr int MBY = 8;    // y rows per block
+// y rows per block: 16 with one cell per lane, 8 with two (FP16 pairs).  At
+// 512x256x256 the FP32 E sweep ran 7% faster with 16 rows than with 8 (the H
+// sweep 0.6%); the paired FP16 sweeps ran 8% slower.
+__host__ __device__ constexpr int tile_rows(int V) { return V == 1 ? 16 : 8; }
 constexpr int MLZ = 67;   // LDS row length: 64 + 3 halo columns
 
 struct MarchSet {
@@ -254,19 +257,21 @@ struct MarchSet {
   int start[7];
   int total, per, seg;
   int bz;  // z extent of a tile: BZ * (cells per lane)
+  int by;  // y extent of a tile: tile_rows(cells per lane)
 };
 
-static MarchSet make_march(const Region* regs, int nreg, int seg, int bz) {
+static MarchSet make_march(const Region* regs, int nreg, int seg, int bz, int by) {
   MarchSet s{};
   s.seg = seg;
   s.bz = bz;
+  s.by = by;
   int acc = 0, m = 0;
   for (int t = 0; t < nreg && m < 6; t++) {
     const Region& r = regs[t];
     if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
     s.r[m] = r;
     s.nbk[m] = (r.k1 - r.k0 + bz - 1) / bz;
-    s.nbj[m] = (r.j1 - r.j0 + MBY - 1) / MBY;
+    s.nbj[m] = (r.j1 - r.j0 + by - 1) / by;
     s.nbs[m] = (r.i1 - r.i0 + seg - 1) / seg;
     s.start[m] = acc;
     acc += s.nbk[m] * s.nbj[m] * s.nbs[m];
@@ -297,7 +302,7 @@ __device__ __forceinline__ bool march_locate(const MarchSet& ms, MarchTile& t) {
   const Region& R = ms.r[r];
   t.r = r;
   t.k0 = R.k0 + bk * ms.bz;
-  t.j0 = R.j0 + bj * MBY;
+  t.j0 = R.j0 + bj * ms.by;
   t.i0 = R.i0 + bs * ms.seg;
   t.i1 = min(t.i0 + ms.seg, R.i1);
   return true;
@@ -403,11 +408,11 @@ template <typename P, typename T> struct PreH {
 // always on: they are per-thread constants.
 // SP: split-field absorbing layer instead of the CPML (one cell per lane).
 template <int STORE, int GEN, int V, bool SP = false>
-__global__ __launch_bounds__(BZ* MBY, 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
+__global__ __launch_bounds__(BZ* tile_rows(V), 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
   using T = typename Raw<STORE>::T;
   using P = typename Vec<STORE, V>::P;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
-  constexpr int BZV = BZ * V, LZ = BZV + 3;
+  constexpr int BZV = BZ * V, LZ = BZV + 3, MBY = tile_rows(V);
   __shared__ float sEx[2][MBY + 3][LZ];  // [row j-j0+2][col k-k0+2], plane i
   __shared__ float rEy[5][MBY][LZ];      // [row j-j0][col k-k0+2], ring of planes
   __shared__ float rEz[5][MBY + 3][BZV]; // [row j-j0+2][col k-k0], ring of planes
@@ -837,14 +842,14 @@ template <> struct ClsWord<1> { using T = uint8_t; };
 template <> struct ClsWord<2> { using T = unsigned short; };
 
 template <int STORE, int GEN, int V, bool SP = false>
-__global__ __launch_bounds__(BZ* MBY, 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
+__global__ __launch_bounds__(BZ* tile_rows(V), 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
                                                          Ade ade, MarchSet ms) {
   using T = typename Raw<STORE>::T;
   using P = typename Vec<STORE, V>::P;
   using CL = typename ClsWord<V>::T;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
   constexpr bool DES = V == 1;  // design-box support (its launches use one cell per lane)
-  constexpr int BZV = BZ * V, LZ = BZV + 3;
+  constexpr int BZV = BZ * V, LZ = BZV + 3, MBY = tile_rows(V);
   __shared__ ClassEntry tbl[kMaxClasses];
   __shared__ float sHx[2][MBY + 3][LZ];  // [row j-j0+1][col k-k0+1], plane i
   __shared__ float rHy[5][MBY][LZ];      // [row j-j0][col k-k0+1], ring of planes
@@ -1192,7 +1197,7 @@ static void check_launch(const char* what) {
 
 // Blocks of one launch that fit on the device at once.
 template <typename K>
-static int resident_blocks(K kernel) {
+static int resident_blocks(K kernel, int threads) {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -1201,7 +1206,7 @@ static int resident_blocks(K kernel) {
     cus = p.multiProcessorCount;
   }
   int per = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BZ * MBY, 0) != hipSuccess || per < 1) per = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
   return per * cus;
 }
 
@@ -1209,14 +1214,14 @@ static int resident_blocks(K kernel) {
 // about kWarm planes (three ring planes loaded synchronously, then the
 // two-plane prefetch filling), and the launch runs in ceil(blocks / slots)
 // rounds.  Minimise rounds * (seg + kWarm).
-static int march_seg(const Region* regs, int nreg, int slots, int bz) {
+static int march_seg(const Region* regs, int nreg, int slots, int bz, int by) {
   constexpr int kWarm = 6;
   long long cols = 0;
   int maxlen = 1;
   for (int t = 0; t < nreg; t++) {
     const Region& r = regs[t];
     if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
-    cols += (long long)((r.k1 - r.k0 + bz - 1) / bz) * ((r.j1 - r.j0 + MBY - 1) / MBY);
+    cols += (long long)((r.k1 - r.k0 + bz - 1) / bz) * ((r.j1 - r.j0 + by - 1) / by);
     maxlen = std::max(maxlen, r.i1 - r.i0);
   }
   int best = maxlen;
@@ -1228,7 +1233,7 @@ static int march_seg(const Region* regs, int nreg, int slots, int bz) {
     for (int t = 0; t < nreg; t++) {
       const Region& r = regs[t];
       if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
-      blocks += (long long)((r.k1 - r.k0 + bz - 1) / bz) * ((r.j1 - r.j0 + MBY - 1) / MBY) *
+      blocks += (long long)((r.k1 - r.k0 + bz - 1) / bz) * ((r.j1 - r.j0 + by - 1) / by) *
                 ((r.i1 - r.i0 + seg - 1) / seg);
     }
     const long long rounds = (blocks + slots - 1) / slots;
@@ -1247,9 +1252,10 @@ template <int STORE, int V, bool SP>
 static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
                   hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
-  const dim3 blk(BZ, MBY, 1);
-  static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP>);
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V), BZ * V);
+  constexpr int by = tile_rows(V);
+  const dim3 blk(BZ, by, 1);
+  static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP>, BZ * by);
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V, by), BZ * V, by);
   if (rg.total > 0) {
     hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
     check_launch("k_update_h");
@@ -1260,9 +1266,10 @@ template <int STORE, int V, bool SP>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                   const Ade& ade, const Region* regs, int nreg, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
-  const dim3 blk(BZ, MBY, 1);
-  static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP>);
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V), BZ * V);
+  constexpr int by = tile_rows(V);
+  const dim3 blk(BZ, by, 1);
+  static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP>, BZ * by);
+  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V, by), BZ * V, by);
   if (rg.total > 0) {
     hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
     check_launch("k_update_e");
