@@ -162,8 +162,9 @@ __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int h
   const bool pin = h >= lo && h <= hi;
   const int ho[4] = {-3, -1, 1, 3};
   const double wt[4] = {1.0, -27.0, 27.0, -1.0};
-  // the four neighbours (odd offsets along a) hold one component
-  const int cq = comp_at(a == 0 ? hx + 1 : hx, a == 1 ? hy + 1 : hy, a == 2 ? hz + 1 : hz);
+  // the four neighbours (odd offsets along a) hold one component, the same
+  // for every cell of a band and component (wavefront-uniform)
+  const int cq = __builtin_amdgcn_readfirstlane(comp_at(a == 0 ? hx + 1 : hx, a == 1 ? hy + 1 : hy, a == 2 ? hz + 1 : hz));
   double acc = 0.0;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -703,14 +704,19 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
                                               Tfsf tf, TfsfBands bands, double tinc) {
   constexpr int HALF = MODE == 1 ? 1 : 0;
   // band and position from the thread index: 32-bit arithmetic (band cells
-  // number far fewer than 2^31; 64-bit division would dominate the kernel)
+  // number far fewer than 2^31; 64-bit division would dominate the kernel).
+  // Each band starts at a multiple of 64 threads, so a wavefront lies in one
+  // band: the band index is found from the wavefront's first thread and the
+  // band tables are read with scalar loads.
   const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned g0 = __builtin_amdgcn_readfirstlane(gid);
   int r = 0;
-  while (r < bands.n && gid >= bands.start[r + 1]) r++;
+  while (r < bands.n && g0 >= bands.start[r + 1]) r++;
   if (r >= bands.n) return;
   const Region& R = bands.r[r];
   const unsigned nk = (unsigned)(R.k1 - R.k0), nj = (unsigned)(R.j1 - R.j0);
   unsigned loc = gid - bands.start[r];
+  if (loc >= nk * nj * (unsigned)(R.i1 - R.i0)) return;  // the band's padding
   const unsigned lk = loc % nk;
   loc /= nk;
   const int k = R.k0 + (int)lk;
@@ -1322,7 +1328,7 @@ static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const C
   b.start[0] = 0;
   for (int r = 0; r < b.n; r++) {
     const Region& R = b.r[r];
-    n += (long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0);
+    n += ((long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0) + 63) / 64 * 64;  // wavefront-aligned
     b.start[r + 1] = (unsigned)std::min<long long>(n, 0xffffffffLL);
     b.ovl[r] = 0;
     for (int q = 0; q < b.n; q++) {
