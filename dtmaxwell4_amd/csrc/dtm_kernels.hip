@@ -239,16 +239,16 @@ __device__ __forceinline__ void cpml_step(float* psi, long long pq, float old, f
 }
 
 // ===================================================== marching sweeps
-// A block owns a tile of 64 (z) x tile_rows(V) (y) columns and marches along x over
+// A block owns a tile of 64 (z) x ROWS (y) columns and marches along x over
 // a segment of planes.  Per thread, the x-neighbours of its column live in a
 // 4-deep register queue (the tilted space-time march of the reference's
 // torre, genAlg.py:420-499, with the rag shuffles of copyrags.cuh replaced
 // by a register queue); the y/z neighbours of the current plane come from a
 // double-buffered LDS tile with a 3-cell halo.  One barrier per plane.
-// y rows per block: 16 with one cell per lane, 8 with two (FP16 pairs).  At
-// 512x256x256 the FP32 E sweep ran 7% faster with 16 rows than with 8 (the H
-// sweep 0.6%); the paired FP16 sweeps ran 8% slower.
-__host__ __device__ constexpr int tile_rows(int V) { return V == 1 ? 16 : 8; }
+// y rows per block (ROWS): 8, or 16 with one cell per lane when the launch
+// then fills the chip as well (see upd_h).  At 512x256x256 the FP32 E sweep
+// ran 7% faster with 16 rows (the H sweep 1%); the paired FP16 sweeps ran 8%
+// slower, and at 400x200x200 the 16-row launch left a fifth of the CUs idle.
 constexpr int MLZ = 67;   // LDS row length: 64 + 3 halo columns
 
 struct MarchSet {
@@ -258,7 +258,7 @@ struct MarchSet {
   int start[7];
   int total, per, seg;
   int bz;  // z extent of a tile: BZ * (cells per lane)
-  int by;  // y extent of a tile: tile_rows(cells per lane)
+  int by;  // y extent of a tile: the kernel's ROWS
 };
 
 static MarchSet make_march(const Region* regs, int nreg, int seg, int bz, int by) {
@@ -408,12 +408,12 @@ template <typename P, typename T> struct PreH {
 // bit 1: y CPML (per row).  The y and z boundary rules and the z CPML are
 // always on: they are per-thread constants.
 // SP: split-field absorbing layer instead of the CPML (one cell per lane).
-template <int STORE, int GEN, int V, bool SP = false>
-__global__ __launch_bounds__(BZ* tile_rows(V), 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
+template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8>
+__global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
   using T = typename Raw<STORE>::T;
   using P = typename Vec<STORE, V>::P;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
-  constexpr int BZV = BZ * V, LZ = BZV + 3, MBY = tile_rows(V);
+  constexpr int BZV = BZ * V, LZ = BZV + 3, MBY = ROWS;
   __shared__ float sEx[2][MBY + 3][LZ];  // [row j-j0+2][col k-k0+2], plane i
   __shared__ float rEy[5][MBY][LZ];      // [row j-j0][col k-k0+2], ring of planes
   __shared__ float rEz[5][MBY + 3][BZV]; // [row j-j0+2][col k-k0], ring of planes
@@ -847,15 +847,15 @@ template <int V> struct ClsWord;
 template <> struct ClsWord<1> { using T = uint8_t; };
 template <> struct ClsWord<2> { using T = unsigned short; };
 
-template <int STORE, int GEN, int V, bool SP = false>
-__global__ __launch_bounds__(BZ* tile_rows(V), 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
+template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8>
+__global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
                                                          Ade ade, MarchSet ms) {
   using T = typename Raw<STORE>::T;
   using P = typename Vec<STORE, V>::P;
   using CL = typename ClsWord<V>::T;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
   constexpr bool DES = V == 1;  // design-box support (its launches use one cell per lane)
-  constexpr int BZV = BZ * V, LZ = BZV + 3, MBY = tile_rows(V);
+  constexpr int BZV = BZ * V, LZ = BZV + 3, MBY = ROWS;
   __shared__ ClassEntry tbl[kMaxClasses];
   __shared__ float sHx[2][MBY + 3][LZ];  // [row j-j0+1][col k-k0+1], plane i
   __shared__ float rHy[5][MBY][LZ];      // [row j-j0][col k-k0+1], ring of planes
@@ -1219,15 +1219,13 @@ static int resident_blocks(K kernel, int threads) {
 // Segment length along x: a block marches seg planes after a warm-up of
 // about kWarm planes (three ring planes loaded synchronously, then the
 // two-plane prefetch filling), and the launch runs in ceil(blocks / slots)
-// rounds.  Minimise rounds * (seg + kWarm).
-static int march_seg(const Region* regs, int nreg, int slots, int bz, int by) {
+// rounds.  Minimise rounds * (seg + kWarm); *cost receives the minimum.
+static int march_seg(const Region* regs, int nreg, int slots, int bz, int by, long long* cost_out = nullptr) {
   constexpr int kWarm = 6;
-  long long cols = 0;
   int maxlen = 1;
   for (int t = 0; t < nreg; t++) {
     const Region& r = regs[t];
     if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) continue;
-    cols += (long long)((r.k1 - r.k0 + bz - 1) / bz) * ((r.j1 - r.j0 + by - 1) / by);
     maxlen = std::max(maxlen, r.i1 - r.i0);
   }
   int best = maxlen;
@@ -1250,36 +1248,72 @@ static int march_seg(const Region* regs, int nreg, int slots, int bz, int by) {
     }
     if (seg <= 4) break;
   }
-  (void)cols;
+  if (cost_out) *cost_out = best_cost;
   return best;
+}
+
+// Tile height of a one-cell-per-lane sweep: 16 rows (one 1024-lane block per
+// CU) unless the 8-row launch (two blocks per CU) has a lower march cost.  A
+// block of either height keeps its CU equally busy per plane, so the costs
+// compare directly.
+static bool use_16_rows(int slots8, int slots16, const Region* regs, int nreg, int bz, int& seg8, int& seg16) {
+  long long c8 = 0, c16 = 0;
+  seg8 = march_seg(regs, nreg, slots8, bz, 8, &c8);
+  seg16 = march_seg(regs, nreg, slots16, bz, 16, &c16);
+  return c16 <= c8;
 }
 
 template <int STORE, int V, bool SP>
 static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
                   hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
-  constexpr int by = tile_rows(V);
-  const dim3 blk(BZ, by, 1);
-  static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP>, BZ * by);
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V, by), BZ * V, by);
-  if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, rg);
-    check_launch("k_update_h");
+  int seg8 = 0, seg16 = 0;
+  if constexpr (V == 1) {
+    static const int slots8 = resident_blocks(k_update_h<STORE, 3, V, SP, 8>, BZ * 8);
+    static const int slots16 = resident_blocks(k_update_h<STORE, 3, V, SP, 16>, BZ * 16);
+    if (use_16_rows(slots8, slots16, regs, nreg, BZ, seg8, seg16)) {
+      MarchSet rg = make_march(regs, nreg, seg16, BZ, 16);
+      if (rg.total <= 0) return;
+      hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 16>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
+                         rg);
+      check_launch("k_update_h");
+      return;
+    }
+  } else {
+    static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP, 8>, BZ * 8);
+    seg8 = march_seg(regs, nreg, slots, BZ * V, 8);
   }
+  MarchSet rg = make_march(regs, nreg, seg8, BZ * V, 8);
+  if (rg.total <= 0) return;
+  hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 8>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, rg);
+  check_launch("k_update_h");
 }
 
 template <int STORE, int V, bool SP>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                   const Ade& ade, const Region* regs, int nreg, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
-  constexpr int by = tile_rows(V);
-  const dim3 blk(BZ, by, 1);
-  static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP>, BZ * by);
-  MarchSet rg = make_march(regs, nreg, march_seg(regs, nreg, slots, BZ * V, by), BZ * V, by);
-  if (rg.total > 0) {
-    hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP>), dim3(rg.per * NXCD), blk, 0, st, g, f, sc, pm, mt, ade, rg);
-    check_launch("k_update_e");
+  int seg8 = 0, seg16 = 0;
+  if constexpr (V == 1) {
+    static const int slots8 = resident_blocks(k_update_e<STORE, 3, V, SP, 8>, BZ * 8);
+    static const int slots16 = resident_blocks(k_update_e<STORE, 3, V, SP, 16>, BZ * 16);
+    if (use_16_rows(slots8, slots16, regs, nreg, BZ, seg8, seg16)) {
+      MarchSet rg = make_march(regs, nreg, seg16, BZ, 16);
+      if (rg.total <= 0) return;
+      hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 16>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
+                         mt, ade, rg);
+      check_launch("k_update_e");
+      return;
+    }
+  } else {
+    static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP, 8>, BZ * 8);
+    seg8 = march_seg(regs, nreg, slots, BZ * V, 8);
   }
+  MarchSet rg = make_march(regs, nreg, seg8, BZ * V, 8);
+  if (rg.total <= 0) return;
+  hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 8>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, mt,
+                     ade, rg);
+  check_launch("k_update_e");
 }
 
 // FP16 storage on an even nz runs two z cells per lane (__half2 words);
