@@ -369,6 +369,13 @@ static int setup_tfsf(dtm_solver* s) {
   for (int a = 0; a < 3; a++) {
     t.pos[a] = c.tfsf.pos[a];
     t.dip[a] = 0.5 * (c.tfsf.box[2 * a] + c.tfsf.box[2 * a + 1]);
+    // The dipole field is singular at the box centre.  The corrections
+    // evaluate it up to 2 half-cells inside a face, so the centre must lie
+    // further in than that on every axis.
+    const double hc = 2.0 * t.dip[a] / d[a];
+    if (t.kind == DTM_TFSF_DIPOLE && t.hlo[a] <= t.hhi[a] && (hc - t.hlo[a] <= 2.5 || t.hhi[a] - hc <= 2.5))
+      return fail(DTM_ERR_ARG, "dipole TF/SF box too small: its centre must lie more than 1.25 cells inside "
+                               "every face (the incident field is singular there)");
   }
   // A plane wave whose k lies along one axis (the other components below
   // 1e-12 |k|, e.g. cos(pi/2) rounding) has a phase that depends on that

@@ -54,3 +54,17 @@ def test_purcell_fp32_matches_fp64_oracle_and_fp16_tracks_fp32():
     e16 = perr(P16, P32)
     print("fp16 vs fp32", e16)
     assert np.isfinite(P16).all() and e16 < 1.0
+
+
+def test_dipole_box_too_small_for_the_singular_field_is_rejected():
+    """A dipole TF/SF box of half-width 1 cell would evaluate the incident
+    field at the dipole itself (1/R^3): dtm_create refuses it; 2 cells is fine."""
+    from dtmaxwell4_amd.solver import Solver
+    w = configs.drude_lorentz_purcell(n=40, npml=8, radius=6, gap=4, nfreq=4, steps=10, cells_per_lambda=20,
+                                      box_half=1)
+    with pytest.raises(abi.DTMError):
+        Solver(w.cfg)
+    w = configs.drude_lorentz_purcell(n=40, npml=8, radius=6, gap=4, nfreq=4, steps=10, cells_per_lambda=20,
+                                      box_half=2)
+    s = Solver(w.cfg)
+    s.close()
