@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-GROUPS_TWOPASS = ("H sweep", "H sweep y-CPML rows", "E sweep", "E sweep y-CPML rows", "TF/SF+ADE+sources+probes")
+GROUPS_TWOPASS = ("H sweep", "-", "E sweep", "-", "TF/SF+ADE+sources+probes")
 GROUPS_FUSED = ("fused step", "-", "-", "-", "TF/SF+ADE+sources+probes")
 # Algorithmic bytes per cell.  Two-pass half-step sweep: read 3E+3H, write 3
 # components (+1 class byte for E).  Fused step (H and E in one pass): read

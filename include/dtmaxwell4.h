@@ -252,9 +252,11 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
 /* Memory scheme of dtm_adjoint_design_gradient.  every = 0 (default) keeps
  * the design-box E of every step.  every = K > 0 keeps the whole time-stepped
  * state every K steps instead and, segment by segment on the way back,
- * recomputes each K steps forward to rebuild their design-box E: device memory
+ * recomputes each K steps forward to rebuild their design-box E (the last
+ * segment's are taken during the forward pass): device memory
  * (ceil(nsteps/K) + 1) states plus K+1 design-box snapshots, for one more
- * forward pass.  For grids whose design box is a large part of the domain.
+ * forward pass less one segment.  For grids whose design box is a large part
+ * of the domain.
  * No counterpart in the reference (it has no adjoint). */
 int dtm_adjoint_set_checkpoints(dtm_solver* s, int every);
 
@@ -265,9 +267,7 @@ int dtm_time_steps(dtm_solver* s, int nsteps, double* ms);
 const char* dtm_kernel_name(const dtm_solver* s);
 /* Per-launch HIP-event timing of the sweep kernels, recorded on the solver's
  * stream while enabled.  Groups: 0 H sweep (fused schedule: the whole step),
- * 1 unused, 2 E sweep, 3 E sweep of the y-CPML rows (only with two FP16
- * cells per lane, whose group 2 then holds the other rows), 4 TF/SF, ADE,
- * sources and probes.
+ * 1 unused, 2 E sweep, 3 unused, 4 TF/SF, ADE, sources and probes.
  * dtm_kernel_times returns the summed ms and launch counts per group and
  * resets them. */
 int dtm_set_kernel_timing(dtm_solver* s, int on);
