@@ -71,6 +71,35 @@ __global__ __launch_bounds__(256) void k_adj_inject(Fields<0> f, const long long
   f.E[2][q[m]] += (float)(gJ * inv_eps[m]);
 }
 
+// k_adj_inject at step n and k_adj_grad of step n-1 in one launch: the
+// first nbg blocks accumulate the gradient, the rest inject.  Only used when
+// no monitor lies in the design box, so the two touch disjoint cells.
+__global__ __launch_bounds__(256) void k_adj_inject_grad(Grid g, Fields<0> f, Materials mt, const float* e0,
+                                                         const float* e1, double* acc, unsigned nbg,
+                                                         const long long* q, const float* inv_eps, const double* F,
+                                                         int nmon, double w, double dt, long long n) {
+  if (blockIdx.x < nbg) {
+    const long long nd = (long long)mt.d_ni * mt.d_nj * mt.d_nk;
+    const long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (b >= nd) return;
+    const int k = mt.d_k0 + (int)(b % mt.d_nk);
+    const int j = mt.d_j0 + (int)((b / mt.d_nk) % mt.d_nj);
+    const int i = mt.d_i0 + (int)(b / ((long long)mt.d_nk * mt.d_nj));
+    const long long qq = i * g.sx + j * g.sy + k;
+    const double eps = 1.0 / (double)mt.d_inv_eps[b];
+    double sum = 0.0;
+    for (int c = 0; c < 3; c++) sum += (double)f.E[c][qq] * ((double)e1[c * nd + b] - (double)e0[c * nd + b]);
+    acc[b] += eps * eps * sum;
+    return;
+  }
+  const int m = (blockIdx.x - nbg) * blockDim.x + threadIdx.x;
+  if (m >= nmon) return;
+  double sn, cn;
+  sincos(w * (double)n * dt, &sn, &cn);
+  const double gJ = 2.0 * dt * (F[2 * m] * cn + F[2 * m + 1] * sn);
+  f.E[2][q[m]] += (float)(gJ * inv_eps[m]);
+}
+
 void check(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) fprintf(stderr, "dtm: launch of %s failed: %s\n", what, hipGetErrorString(e));
@@ -99,6 +128,16 @@ void launch_adj_grad(const Grid& g, const void* fields, const Materials& mt, con
   hipLaunchKernelGGL(k_adj_grad, dim3(blocks_for(nd)), dim3(256), 0, st, g, *static_cast<const Fields<0>*>(fields), mt,
                      e0, e1, acc);
   check("k_adj_grad");
+}
+
+void launch_adj_inject_grad(const Grid& g, const void* fields, const Materials& mt, const float* e0, const float* e1,
+                            double* acc, const long long* q, const float* inv_eps, const double* F, int nmon,
+                            double w, double dt, long long n, hipStream_t st) {
+  const long long nd = (long long)mt.d_ni * mt.d_nj * mt.d_nk;
+  const unsigned nbg = blocks_for(nd);
+  hipLaunchKernelGGL(k_adj_inject_grad, dim3(nbg + blocks_for(nmon)), dim3(256), 0, st, g,
+                     *static_cast<const Fields<0>*>(fields), mt, e0, e1, acc, nbg, q, inv_eps, F, nmon, w, dt, n);
+  check("k_adj_inject_grad");
 }
 
 void launch_adj_inject(const void* fields, const long long* q, const float* inv_eps, const double* F, int nmon,

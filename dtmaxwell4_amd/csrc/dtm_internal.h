@@ -217,6 +217,11 @@ void launch_adj_grad(const Grid& g, const void* fields, const Materials& mt, con
                      double* acc, hipStream_t st);
 void launch_adj_inject(const void* fields, const long long* q, const float* inv_eps, const double* F, int nmon,
                        double w, double dt, long long n, hipStream_t st);
+// launch_adj_inject at step n, then launch_adj_grad (of step n-1), in one
+// launch; only for monitors outside the design box.
+void launch_adj_inject_grad(const Grid& g, const void* fields, const Materials& mt, const float* e0, const float* e1,
+                            double* acc, const long long* q, const float* inv_eps, const double* F, int nmon,
+                            double w, double dt, long long n, hipStream_t st);
 void launch_tfsf_tables(const Tfsf& tf, double* out, long long n0, int nsteps, double dt, hipStream_t st);
 void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                  const Materials& mt, const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double t_inc,

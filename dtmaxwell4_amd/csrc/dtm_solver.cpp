@@ -1165,14 +1165,17 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
   const long long nd = (long long)mt.d_ni * mt.d_nj * mt.d_nk;
   std::vector<long long> q(nmon);
   std::vector<float> ie(nmon);
+  bool mon_in_design = false;  // a monitor inside the design box keeps injection and gradient apart
   for (int m = 0; m < nmon; m++) {
     const int i = mon_ijk[3 * m], j = mon_ijk[3 * m + 1], k = mon_ijk[3 * m + 2];
     if (i < 0 || i >= c.nx || j < 0 || j >= c.ny || k < 0 || k >= c.nz) return fail(DTM_ERR_ARG, "monitor outside grid");
     q[m] = ((long long)i * c.ny + j) * c.nz + k;
     ie[m] = s->classes[s->cls_host[q[m]]].inv_eps[2];
     const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
-    if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk)
+    if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) {
       ie[m] = s->design_host[((size_t)di * mt.d_nj + dj) * mt.d_nk + dk];
+      mon_in_design = true;
+    }
   }
   // History mode keeps the design-box E of every step (nsteps+1 snapshots).
   // Checkpoint mode (interval K) keeps the whole state every K steps and,
@@ -1282,11 +1285,23 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
         if (rc || (rc = load_state(s, stash, adj))) break;
         s->step = adj_step;
       }
+      // gradient of step n, one backward step, injection at step n; the
+      // injection and the next step's gradient share a launch when they
+      // touch disjoint cells
+      launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n1 - 1 - n0) * snap, hist + (size_t)(n1 - n0) * snap,
+                      acc, s->stream);
       for (int n = n1 - 1; n >= n0 && !rc; n--) {
-        launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n - n0) * snap, hist + (size_t)(n - n0 + 1) * snap,
-                        acc, s->stream);
         rc = one_step(s);
+        const bool next = n - 1 >= n0;  // the gradient of step n-1 follows in this segment
+        if (n >= 1 && next && !mon_in_design) {
+          launch_adj_inject_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n - 1 - n0) * snap,
+                                 hist + (size_t)(n - n0) * snap, acc, hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
+          continue;
+        }
         if (n >= 1) launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
+        if (next)
+          launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n - 1 - n0) * snap, hist + (size_t)(n - n0) * snap,
+                          acc, s->stream);
       }
     }
     s->adjoint = false;
