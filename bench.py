@@ -27,10 +27,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 GROUPS_TWOPASS = ("H sweep", "-", "E sweep", "-", "TF/SF+ADE+sources+probes")
 GROUPS_FUSED = ("fused step", "-", "-", "-", "TF/SF+ADE+sources+probes")
-# Algorithmic bytes per cell.  Two-pass half-step sweep: read 3E+3H, write 3
-# components (+1 class byte for E).  Fused step (H and E in one pass): read
-# 3E+3H, write 3E+3H, read the class byte.  Each CPML slab a cell lies in adds
-# two FP32 psi read and written per half-step (16 B; 32 B per fused step).
+# Algorithmic bytes per cell, the roofline basis: two-pass half-step sweep:
+# read 3E+3H, write 3 components (+1 class byte for E): 36 / 37 B in FP32.
+# Fused step (H and E in one pass): read 3E+3H, write 3E+3H, read the class
+# byte: 49 B.  The CPML psi (two FP32 values read and written per half-step
+# for each slab a cell lies in: 16 B; 32 B per fused step) is region-local
+# and reported beside it (psi_bytes, roofline.extra_bytes), not in it.
 WORDS_PER_HALF = 9
 WORDS_FUSED = 12
 PSI_BYTES_HALF = 2 * 2 * 4
@@ -106,13 +108,17 @@ def gpu_run(storage, sweep, steps, warmup, nx, ny, nz):
             continue
         avg = gms[g] / gcnt[g]
         gc = s.group_cells(g)
+        # Yee bytes (the roofline basis) and the CPML psi bytes, reported apart
         if fused:
-            byts = gc * (WORDS_FUSED * esz + 1 + 2 * PSI_BYTES_HALF * pfrac)
+            byts = gc * (WORDS_FUSED * esz + 1)
+            psib = gc * 2 * PSI_BYTES_HALF * pfrac
         else:
-            byts = gc * (WORDS_PER_HALF * esz + (1 if g in (2, 3) else 0) + PSI_BYTES_HALF * pfrac)
+            byts = gc * (WORDS_PER_HALF * esz + (1 if g in (2, 3) else 0))
+            psib = gc * PSI_BYTES_HALF * pfrac
         groups[names[g]] = dict(avg_ms=avg, launches=int(gcnt[g]), cells=int(gc),
-                                algorithmic_bytes=int(byts),
-                                gbs=(byts / (avg * 1e-3) / 1e9) if (gc and avg > 0) else None)
+                                algorithmic_bytes=int(byts), psi_bytes=int(psib),
+                                gbs=(byts / (avg * 1e-3) / 1e9) if (gc and avg > 0) else None,
+                                gbs_with_psi=((byts + psib) / (avg * 1e-3) / 1e9) if (gc and avg > 0) else None)
     dom = max((g for g in groups if groups[g]["cells"]), key=lambda g: groups[g]["avg_ms"] * groups[g]["launches"])
     bpc = (WORDS_FUSED * esz + 1) if fused else (2 * WORDS_PER_HALF * esz + 1)
     res = dict(gcells=val, ms=ms, wall_s=wall, groups=groups, dominant=dom, kernel=s.kernel_name,
@@ -203,9 +209,30 @@ def purcell_run(steps):
     P32, P16 = curves["fp32"], curves["fp16"]
     out["fp16_vs_fp32_error"] = float(np.max(np.abs(P16 - P32)) / np.max(np.abs(P32)))
     out["P_fp32_range"] = [float(P32.min()), float(P32.max())]
+    # against the FP64 oracle's spectrum of the same workload (fixture written
+    # by tests/golden/make_fullsize_fixtures.py: the oracle needs hours here)
+    fx = os.path.join(ROOT, "tests", "golden", "fullsize_drude_lorentz_purcell.npz")
+    if steps == 4000 and os.path.exists(fx):
+        d = np.load(fx)["dft"]
+        w = configs.drude_lorentz_purcell(steps=steps)
+        Fd = purcell.dipole_spectrum(w.cfg.tfsf, w.omegas, steps, w.cfg.dt)
+        P64 = purcell.purcell_factor(w.omegas, d[0, :, 0] + 1j * d[0, :, 1], Fd)
+        for name, P in (("fp32", P32), ("fp16", P16)):
+            out[f"{name}_vs_fp64_error"] = float(np.max(np.abs(P - P64)) / np.max(np.abs(P64)))
+        out["fp64_reference"] = os.path.relpath(fx, ROOT)
     out["workload"] = (f"Drude+Lorentz sphere r=40 in 384^3, dipole TF/SF 4 cells from the surface, "
                        f"{len(P32)} DFT bins, {steps} steps")
     return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(nx, ny, nz, budget_s=15.0):
@@ -226,7 +253,8 @@ def cpu_baseline(nx, ny, nz, budget_s=15.0):
     el = time.perf_counter() - t
     o.close()
     cells = nx * ny * nz
-    return dict(value=cells * n / el / 1e9, unit="Gcells/s", cores=threads, kind="port",
+    return dict(value=cells * n / el / 1e9, unit="Gcells/s", cores=threads, kind="port", cpu_model=cpu_model(),
+                host_cpus=os.cpu_count(),
                 sample=f"plane_wave_sphere {nx}x{ny}x{nz}, {n} full steps after 3 untimed steps "
                        f"({el:.1f} s), FP32 OpenMP restatement of the same update equations")
 
@@ -276,7 +304,8 @@ def main():
     pur = None
     if not args.no_purcell:
         pur = purcell_run(args.purcell_steps)
-        log(f"[bench] purcell fp16 vs fp32 error {pur['fp16_vs_fp32_error']:.3e}")
+        log(f"[bench] purcell fp16 vs fp32 error {pur['fp16_vs_fp32_error']:.3e}, vs fp64 "
+            f"{pur.get('fp16_vs_fp64_error')}")
     cpu = None
     if not args.no_cpu:
         try:
@@ -311,13 +340,17 @@ def main():
                      "traffic": None if tr is None else round(tr["bytes"]),
                      "traffic_source": None if tr is None else tr["source"],
                      "kernel": r32["kernel"] + " " + r32["dominant"], "avg_launch_ms": dom["avg_ms"],
-                     "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "cells_per_launch": dom["cells"]},
+                     "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "cells_per_launch": dom["cells"],
+                     "extra_bytes": dom["psi_bytes"], "extra": "CPML psi bytes per launch (not in achieved)",
+                     "achieved_with_extra": round(dom["gbs_with_psi"], 1)},
         "effective_yee_gbs": round(r32["gcells"] * 72, 1),
         "effective_yee_frac": round(r32["gcells"] * 72 / HBM_PEAK_GBS, 4),
         "fp16_store": None if r16 is None else {
             "value": round(r16["gcells"], 3), "unit": "Gcells/s", "speedup_vs_fp32": round(r16["gcells"] / r32["gcells"], 3),
             "dominant_kernel_gbs": r16["groups"][r16["dominant"]]["gbs"],
-            "purcell_error": None if pur is None else pur["fp16_vs_fp32_error"]},
+            "purcell_error": None if pur is None else pur.get("fp16_vs_fp64_error", pur["fp16_vs_fp32_error"]),
+            "purcell_error_basis": None if pur is None else (
+                "FP64 oracle" if "fp16_vs_fp64_error" in pur else "FP32 storage")},
         "purcell": pur,
         "other_schedule_fp32": None if ralt is None else {
             "kernel": ralt["kernel"], "value": round(ralt["gcells"], 3), "unit": "Gcells/s",

@@ -10,10 +10,11 @@ from . import abi  # noqa: F401
 
 
 def __getattr__(name):
+    # submodules through importlib: `from . import X` would consult this hook
+    # again before X is imported
+    import importlib
     if name == "Solver":
-        from .solver import Solver
-        return Solver
+        return importlib.import_module(".solver", __name__).Solver
     if name == "DTmxw":
-        from . import DTmxw
-        return DTmxw
+        return importlib.import_module(".DTmxw", __name__)
     raise AttributeError(name)

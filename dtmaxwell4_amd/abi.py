@@ -159,10 +159,18 @@ def load_library():
         "dtm_sync": (C.c_int, [P]),
         "dtm_probe_count": (C.c_int, [P]),
         "dtm_probe_read": (C.c_int, [P, P, C.c_int]),
+        "dtm_probe_last": (C.c_int, [P, P]),
         "dtm_dft_read": (C.c_int, [P, P]),
         "dtm_field_read": (C.c_int, [P, C.c_int, P]),
         "dtm_field_write": (C.c_int, [P, C.c_int, P]),
         "dtm_energy": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "dtm_energy_box": (C.c_int, [P] + [C.c_int] * 6 + [C.POINTER(C.c_double)]),
+        "dtm_add_flux_box": (C.c_int, [P] + [C.c_int] * 6),
+        "dtm_flux_read": (C.c_int, [P, P, C.c_int]),
+        "dtm_flux_total": (C.c_int, [P, P]),
+        "dtm_write_arr": (C.c_int, [P, C.c_int, C.c_char_p]),
+        "dtm_save_state": (C.c_int, [P, C.c_char_p]),
+        "dtm_load_state": (C.c_int, [P, C.c_char_p]),
         "dtm_adjoint_design_gradient": (C.c_int, [P, C.c_int, C.c_double, C.c_int, P, C.POINTER(C.c_double), P]),
         "dtm_adjoint_set_checkpoints": (C.c_int, [P, C.c_int]),
         "dtm_time_steps": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),

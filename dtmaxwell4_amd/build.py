@@ -13,7 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libdtmaxwell4.so")
-SOURCES = ["dtm_kernels.hip", "dtm_torre.hip", "dtm_adjoint.hip", "dtm_solver.cpp", "dtm_legacy.cpp"]
+SOURCES = ["dtm_kernels.hip", "dtm_torre.hip", "dtm_adjoint.hip", "dtm_flux.hip", "dtm_solver.cpp", "dtm_legacy.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DTM_ARCH", "gfx950")
 

@@ -100,10 +100,7 @@ __global__ __launch_bounds__(256) void k_adj_inject_grad(Grid g, Fields<0> f, Ma
   f.E[2][q[m]] += (float)(gJ * inv_eps[m]);
 }
 
-void check(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) fprintf(stderr, "dtm: launch of %s failed: %s\n", what, hipGetErrorString(e));
-}
+void check(const char* what) { note_launch(what); }
 
 unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 

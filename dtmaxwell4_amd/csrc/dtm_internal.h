@@ -192,12 +192,40 @@ struct Probes {
   long long step;
 };
 
+// Poynting-flux faces (dtm_flux.hip).  Face normal to axis a on the H plane
+// pa, tangential ranges [b0,b1] x [c0,c1] (cells, b = a+1, c = a+2 mod 3);
+// `nodes` = (b1-b0+1)(c1-c0) + (b1-b0)(c1-c0+1) face nodes, whose E^n
+// snapshot starts at snap0; area = d_b d_c; sign = -1 (low face) or +1.
+constexpr int kMaxFluxBoxes = 8;
+struct FluxFace {
+  int a, pa, b0, b1, c0, c1;
+  int nodes, snap0;
+  double area, sign;
+};
+struct FluxSet {
+  int nface;
+  FluxFace face[6 * kMaxFluxBoxes];
+  float* snap;     // face-averaged tangential E^n of every node
+  double* series;  // [row][face]
+  double* total;   // [face], running sum of flux * dt
+};
+
 struct Soft {
   int n;
   int comp[16];
   long long q[16];
   float val[16];  // dt * inv_eps * J(t)
 };
+
+// Launch status of the calling thread.  Every launcher calls note_launch
+// right after its hipLaunchKernelGGL; the first failure (with the kernel's
+// name) is kept until take_launch_error hands it to the C-ABI entry point,
+// which then returns DTM_ERR_DEVICE.  A launcher that refuses a launch
+// (a size it cannot index) records it with fail_launch.
+void note_launch(const char* kernel);
+void fail_launch(const char* kernel, const char* why);
+// 1 and the message when a failure was recorded since the last call (cleared).
+int take_launch_error(char* msg, int cap);
 
 // Launchers (dtm_kernels.hip).  All enqueue on `st`; each sweep is one
 // launch of the kernel variant with every boundary rule over `regs`.
@@ -240,8 +268,16 @@ void launch_soft(int store, const Grid& g, const void* fields, const Scales& sc,
                  hipStream_t st);
 void launch_record(int store, const Grid& g, const void* fields, const Scales& sc, const Probes& pr,
                    hipStream_t st);
-void launch_energy(int store, const Grid& g, const void* fields, const Scales& sc,
-                   const Materials& mt, double* out, hipStream_t st);
+// Flux of the step n -> n+1: launch_flux_snap before it, launch_flux after it
+// (dtm_flux.hip).
+void launch_flux_snap(int store, const Grid& g, const void* fields, const Scales& sc, const FluxSet& fs,
+                      hipStream_t st);
+void launch_flux(int store, const Grid& g, const void* fields, const Scales& sc, const FluxSet& fs, int row,
+                 double dt, hipStream_t st);
+// 1/2 sum (eps E^2 + H^2) dv over a box in FP64: one partial per x plane of
+// the box in part[0 .. i1-i0), then their fixed-order sum into *out.
+void launch_energy_box(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt,
+                       const Region& bx, double* part, double dv, double* out, hipStream_t st);
 void launch_to_float(int store, const void* src, float scale_inv, float* dst, long long n,
                      hipStream_t st);
 void launch_from_float(int store, const float* src, float scale, void* dst, long long n,

@@ -1164,28 +1164,6 @@ __global__ void k_record(Grid g, Fields<STORE> f, Scales sc, Probes pr) {
   }
 }
 
-template <int STORE>
-__global__ void k_energy(Grid g, Fields<STORE> f, Scales sc, Materials mt, double* out) {
-  const long long n = (long long)g.nx * g.ny * g.nz;
-  double acc = 0.0;
-  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
-    const ClassEntry& ce = mt.table[mt.cls ? mt.cls[q] : 0];
-    for (int c = 0; c < 3; c++) {
-      const float e = ld(f.E[c], q, sc.ie[c]);
-      const float h = ld(f.H[c], q, sc.ih[c]);
-      acc += (double)e * e / ce.inv_eps[c] + (double)h * h;
-    }
-  }
-  __shared__ double red[256];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(out, red[0]);
-}
-
 __global__ void k_to_float_h(const __half* src, float inv, float* dst, long long n) {
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
     dst[q] = __half2float(src[q]) * inv;
@@ -1196,10 +1174,30 @@ __global__ void k_from_float_h(const float* src, float s, __half* dst, long long
 }
 
 // ============================================================ launchers
-static void check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) fprintf(stderr, "dtm: launch of %s failed: %s\n", what, hipGetErrorString(e));
+// First failed or refused launch of this thread since the last
+// take_launch_error (kernel name and reason).
+static thread_local char t_launch_err[256];
+static thread_local bool t_launch_failed = false;
+
+void fail_launch(const char* kernel, const char* why) {
+  if (t_launch_failed) return;  // keep the first
+  snprintf(t_launch_err, sizeof(t_launch_err), "launch of %s failed: %s", kernel, why);
+  t_launch_failed = true;
 }
+
+void note_launch(const char* kernel) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fail_launch(kernel, hipGetErrorString(e));
+}
+
+int take_launch_error(char* msg, int cap) {
+  if (!t_launch_failed) return 0;
+  if (msg && cap > 0) snprintf(msg, (size_t)cap, "%s", t_launch_err);
+  t_launch_failed = false;
+  return 1;
+}
+
+static void check_launch(const char* what) { note_launch(what); }
 
 // Blocks of one launch that fit on the device at once.
 template <typename K>
@@ -1372,8 +1370,8 @@ static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const C
     }
   }
   if (n <= 0) return;
-  if (n >= (1LL << 31)) {
-    fprintf(stderr, "dtm: TF/SF bands hold %lld cells (limit 2^31)\n", n);
+  if (n >= (1LL << 31)) {  // dtm_create rejects such boxes; never index past 2^31 threads
+    fail_launch("k_tfsf", "TF/SF face bands hold 2^31 cells or more");
     return;
   }
   const dim3 grid((unsigned)((n + 255) / 256));
@@ -1434,20 +1432,10 @@ void launch_record(int store, const Grid& g, const void* fields, const Scales& s
   check_launch("k_record");
 }
 
-void launch_energy(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt, double* out,
-                   hipStream_t st) {
-  if (store == 0)
-    hipLaunchKernelGGL(k_energy<0>, dim3(1024), dim3(256), 0, st, g, *static_cast<const Fields<0>*>(fields), sc, mt,
-                       out);
-  else
-    hipLaunchKernelGGL(k_energy<1>, dim3(1024), dim3(256), 0, st, g, *static_cast<const Fields<1>*>(fields), sc, mt,
-                       out);
-  check_launch("k_energy");
-}
-
 void launch_to_float(int store, const void* src, float inv, float* dst, long long n, hipStream_t st) {
   if (store == 0) {
-    (void)hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st);
+    if (hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      fail_launch("field copy", hipGetErrorString(hipGetLastError()));
     return;
   }
   hipLaunchKernelGGL(k_to_float_h, dim3(1024), dim3(256), 0, st, static_cast<const __half*>(src), inv, dst, n);
@@ -1456,7 +1444,8 @@ void launch_to_float(int store, const void* src, float inv, float* dst, long lon
 
 void launch_from_float(int store, const float* src, float s, void* dst, long long n, hipStream_t st) {
   if (store == 0) {
-    (void)hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st);
+    if (hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      fail_launch("field copy", hipGetErrorString(hipGetLastError()));
     return;
   }
   hipLaunchKernelGGL(k_from_float_h, dim3(1024), dim3(256), 0, st, src, s, static_cast<__half*>(dst), n);

@@ -188,13 +188,10 @@ int calcStep(void) {
   }
   // Sensor::write once per calcStep (diamond.cu:473-475, sensor.h:81)
   if (!g_run.sensors.empty()) {
-    const int np = dtm_probe_count(g_run.s);
-    const long long rows = dtm_time_step(g_run.s);
-    std::vector<float> buf((size_t)rows * np);
-    const int r = dtm_probe_read(g_run.s, buf.data(), (int)rows);
-    if (r > 0)
+    std::vector<float> last((size_t)dtm_probe_count(g_run.s));
+    if (dtm_probe_last(g_run.s, last.data()) > 0)
       for (auto& sf : g_run.sensors) {
-        fprintf(sf.f, "%g\n", buf[(size_t)(r - 1) * np + sf.probe]);
+        fprintf(sf.f, "%g\n", last[sf.probe]);
         fflush(sf.f);
       }
   }

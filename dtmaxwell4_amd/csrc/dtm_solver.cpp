@@ -33,6 +33,14 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+// DTM_ERR_DEVICE with the kernel and reason when a launch since the last
+// call failed or was refused (see note_launch).
+static int launch_result() {
+  char m[256];
+  if (take_launch_error(m, sizeof(m))) return fail(DTM_ERR_DEVICE, m);
+  return 0;
+}
+
 #define HIPCHK(expr)                                                                        \
   do {                                                                                      \
     hipError_t _e = (expr);                                                                 \
@@ -95,6 +103,13 @@ struct dtm_solver {
   double* omega_dev = nullptr;
   double* dft_dev = nullptr;
   double* energy_dev = nullptr;
+  double* energy_part = nullptr;  // box energy: one FP64 partial per x plane
+  int energy_part_cap = 0;
+  // Poynting-flux boxes (series rows share the probe series' row counter)
+  std::vector<FluxFace> flux_faces;
+  int flux_snap_n = 0;
+  FluxSet fs{};
+  std::vector<double> flux_host;
   float* tmp_dev = nullptr;  // conversion buffer (FP16 field access)
   long long step = 0;
   bool started = false;
@@ -317,10 +332,12 @@ static void sf_bounds(double bm, double bp, double d, int nh, int& lo, int& hi) 
   }
 }
 
-static int setup_tfsf(dtm_solver* s) {
-  const dtm_config& c = s->cfg;
-  Tfsf& t = s->tf;
+// TF/SF geometry, incident-field constants and face bands of a config:
+// host arithmetic only, so dtm_create validates the source before it
+// touches the device.
+static int tfsf_plan(const dtm_config& c, Tfsf& t, TfsfBands& B) {
   memset(&t, 0, sizeof(t));
+  memset(&B, 0, sizeof(B));
   t.kind = c.tfsf.kind;
   if (t.kind == DTM_TFSF_NONE) return 0;
   if (t.kind != DTM_TFSF_DIPOLE && t.kind != DTM_TFSF_PLANE) return fail(DTM_ERR_ARG, "unknown tfsf kind");
@@ -390,7 +407,6 @@ static int setup_tfsf(dtm_solver* s) {
     if (t.tab_axis >= 0) {
       t.tab_h0 = -8;
       t.tab_n = 2 * n[t.tab_axis] + 16;
-      if (int rc = dalloc(s, &s->tab_base, sizeof(double) * 2 * kTableSteps * t.tab_n)) return rc;
       // incident component = P[c] * amp*f0(phase) (signal.hpp:28-49 with
       // E_phi = Imp cos(psi) s, E_theta = Imp sin(psi) s, H_theta = -cos(psi) s,
       // H_phi = sin(psi) s)
@@ -403,7 +419,6 @@ static int setup_tfsf(dtm_solver* s) {
     }
   }
   // face bands, clamped to the grid; lo/hi merged when they meet
-  TfsfBands& B = s->bands;
   B.n = 0;
   auto clampr = [](int lo, int hi, int n, int& o0, int& o1) {
     o0 = std::max(lo, 0);
@@ -433,6 +448,22 @@ static int setup_tfsf(dtm_solver* s) {
       B.n++;
     }
   }
+  // the correction kernel indexes band cells with 32-bit thread ids, each
+  // band padded to whole wavefronts
+  long long nthr = 0;
+  for (int r = 0; r < B.n; r++) {
+    const Region& R = B.r[r];
+    nthr += ((long long)(R.i1 - R.i0) * (R.j1 - R.j0) * (R.k1 - R.k0) + 63) / 64 * 64;
+  }
+  if (nthr >= (1LL << 31))
+    return fail(DTM_ERR_LIMIT, "TF/SF face bands hold 2^31 cells or more (the correction kernel's index range)");
+  return 0;
+}
+
+static int setup_tfsf(dtm_solver* s) {
+  if (int rc = tfsf_plan(s->cfg, s->tf, s->bands)) return rc;
+  if (s->tf.kind != DTM_TFSF_NONE && s->tf.tab_axis >= 0)
+    if (int rc = dalloc(s, &s->tab_base, sizeof(double) * 2 * kTableSteps * s->tf.tab_n)) return rc;
   return 0;
 }
 
@@ -576,8 +607,24 @@ static void swap_buffers(dtm_solver* s) {
     }
 }
 
-// Build the class table from per-component material ids.
+static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]);
+
+// Build the class table from per-component material ids.  A volume that is
+// rejected leaves the solver's current materials in place.
 static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
+  auto classes = s->classes;
+  auto class_mats = s->class_mats;
+  auto cls_host = s->cls_host;
+  const int rc = build_classes_into(s, m);
+  if (rc) {
+    s->classes.swap(classes);
+    s->class_mats.swap(class_mats);
+    s->cls_host.swap(cls_host);
+  }
+  return rc;
+}
+
+static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]) {
   const dtm_config& c = s->cfg;
   const size_t n = (size_t)c.nx * c.ny * c.nz;
   std::unordered_map<uint32_t, int> ids;
@@ -631,6 +678,21 @@ static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
       bb[5] = std::max(bb[5], k);
     }
   }
+  if (s->mt.d_on && any_special) {  // a design box set earlier must stay on plain cells
+    const Materials& mt = s->mt;
+    for (int i = mt.d_i0; i < mt.d_i0 + mt.d_ni; i++)
+      for (int j = mt.d_j0; j < mt.d_j0 + mt.d_nj; j++)
+        for (int k = mt.d_k0; k < mt.d_k0 + mt.d_nk; k++)
+          if (s->classes[s->cls_host[((size_t)i * c.ny + j) * c.nz + k]].special)
+            return fail(DTM_ERR_ARG, "the design box overlaps a dispersive or gyrotropic cell");
+  }
+  // soft sources must sit in plain cells
+  for (int q = 0; q < c.nsoft; q++) {
+    const dtm_soft_source& so = c.soft[q];
+    const size_t idx = ((size_t)so.i * c.ny + so.j) * c.nz + so.k;
+    if (s->classes[s->cls_host[idx]].special)
+      return fail(DTM_ERR_ARG, "soft source placed in a dispersive/gyrotropic cell");
+  }
   const int ncls = (int)s->classes.size();
   if (!s->table_dev)
     if (int rc = dalloc(s, &s->table_dev, sizeof(ClassEntry) * kMaxClasses)) return rc;
@@ -682,13 +744,6 @@ static int build_classes(dtm_solver* s, const uint8_t* const m[3]) {
     A.mats = s->ademats_dev;
     A.gmats = s->gyromats_dev;
   }
-  // soft sources must sit in plain cells
-  for (int q = 0; q < c.nsoft; q++) {
-    const dtm_soft_source& so = c.soft[q];
-    const size_t idx = ((size_t)so.i * c.ny + so.j) * c.nz + so.k;
-    if (s->classes[s->cls_host[idx]].special)
-      return fail(DTM_ERR_ARG, "soft source placed in a dispersive/gyrotropic cell");
-  }
   return 0;
 }
 
@@ -729,6 +784,11 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
     if (so.comp < 0 || so.comp > 2 || so.i < 0 || so.i >= c.nx || so.j < 0 || so.j >= c.ny || so.k < 0 ||
         so.k >= c.nz)
       return fail(DTM_ERR_ARG, "soft source outside the grid");
+  }
+  {  // the source is checked before any device call
+    Tfsf t;
+    TfsfBands b;
+    if (int rc = tfsf_plan(c, t, b)) return rc;
   }
   dtm_solver* s = new dtm_solver();
   s->cfg = c;
@@ -896,13 +956,22 @@ int dtm_set_dft(dtm_solver* s, int nfreq, const double* omegas) {
 static int start(dtm_solver* s) {
   if (s->started) return 0;
   const int np = (int)s->probes.size();
+  const int nf = (int)s->flux_faces.size();
+  if (np > 0 || nf > 0) s->series_cap = std::max(16, std::min(4096, (1 << 22) / std::max(np, nf)));
   if (np > 0) {
     if (int rc = dalloc(s, &s->probes_dev, sizeof(ProbeDesc) * np)) return rc;
     HIPCHK(hipMemcpy(s->probes_dev, s->probes.data(), sizeof(ProbeDesc) * np, hipMemcpyHostToDevice));
-    s->series_cap = std::max(16, std::min(4096, (1 << 22) / np));
     if (int rc = dalloc(s, &s->series_dev, sizeof(float) * (size_t)s->series_cap * np)) return rc;
     if (s->nfreq > 0)
       if (int rc = dalloc(s, &s->dft_dev, sizeof(double) * 2 * (size_t)np * s->nfreq)) return rc;
+  }
+  if (nf > 0) {
+    FluxSet& fs = s->fs;
+    fs.nface = nf;
+    for (int q = 0; q < nf; q++) fs.face[q] = s->flux_faces[q];
+    if (int rc = dalloc(s, &fs.snap, sizeof(float) * (size_t)s->flux_snap_n)) return rc;
+    if (int rc = dalloc(s, &fs.series, sizeof(double) * (size_t)s->series_cap * nf)) return rc;
+    if (int rc = dalloc(s, &fs.total, sizeof(double) * nf)) return rc;
   }
   s->started = true;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -911,11 +980,20 @@ static int start(dtm_solver* s) {
 
 static int flush_series(dtm_solver* s) {
   const int np = (int)s->probes.size();
-  if (np == 0 || s->series_row == 0) return 0;
-  const size_t old = s->series_host.size();
-  s->series_host.resize(old + (size_t)s->series_row * np);
-  HIPCHK(hipMemcpyAsync(s->series_host.data() + old, s->series_dev, sizeof(float) * (size_t)s->series_row * np,
-                        hipMemcpyDeviceToHost, s->stream));
+  const int nf = s->fs.nface;
+  if ((np == 0 && nf == 0) || s->series_row == 0) return 0;
+  if (np > 0) {
+    const size_t old = s->series_host.size();
+    s->series_host.resize(old + (size_t)s->series_row * np);
+    HIPCHK(hipMemcpyAsync(s->series_host.data() + old, s->series_dev, sizeof(float) * (size_t)s->series_row * np,
+                          hipMemcpyDeviceToHost, s->stream));
+  }
+  if (nf > 0) {
+    const size_t old = s->flux_host.size();
+    s->flux_host.resize(old + (size_t)s->series_row * nf);
+    HIPCHK(hipMemcpyAsync(s->flux_host.data() + old, s->fs.series, sizeof(double) * (size_t)s->series_row * nf,
+                          hipMemcpyDeviceToHost, s->stream));
+  }
   HIPCHK(hipStreamSynchronize(s->stream));
   s->series_row = 0;
   return 0;
@@ -969,6 +1047,8 @@ static int one_step(dtm_solver* s) {
   const long long n = s->step;
   s->tf.active = (!s->adjoint && tfsf_gate(s, n)) ? 1 : 0;
   const void* F = fields_ptr(s);
+  const bool rec = !s->adjoint && !s->replay;  // probes and flux record forward steps only
+  if (rec && s->fs.nface > 0) launch_flux_snap(s->store, s->g, F, s->sc, s->fs, s->stream);
   const double th = (double)n * c.dt, te = ((double)n + 0.5) * c.dt;
   if (s->torre) {
     // TF/SF on H^{n-1/2} before the fused step (exact: the step is linear)
@@ -1028,7 +1108,9 @@ static int one_step(dtm_solver* s) {
     }
     launch_soft(s->store, s->g, F, s->sc, so, s->stream);
   }
-  const int np = (s->adjoint || s->replay) ? 0 : (int)s->probes.size();
+  const int np = rec ? (int)s->probes.size() : 0;
+  const int nf = rec ? s->fs.nface : 0;
+  if (nf > 0) launch_flux(s->store, s->g, F, s->sc, s->fs, s->series_row, c.dt, s->stream);
   if (np > 0) {
     Probes pr{};
     pr.n = np;
@@ -1043,7 +1125,8 @@ static int one_step(dtm_solver* s) {
     launch_record(s->store, s->g, F, s->sc, pr, s->stream);
   }
   ev_end(s);
-  if (np > 0) {
+  if (int rc = launch_result()) return rc;
+  if (np > 0 || nf > 0) {
     s->series_row++;
     if (s->series_row == s->series_cap)
       if (int rc = flush_series(s)) return rc;
@@ -1057,9 +1140,7 @@ int dtm_step(dtm_solver* s, int nsteps) {
   if (int rc = start(s)) return rc;
   for (int t = 0; t < nsteps; t++)
     if (int rc = one_step(s)) return rc;
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(DTM_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
-  return 0;
+  return launch_result();
 }
 
 long long dtm_time_step(const dtm_solver* s) { return s ? s->step : -1; }
@@ -1081,6 +1162,21 @@ int dtm_probe_read(dtm_solver* s, float* out, int nsteps_max) {
   const long long r = std::min<long long>(rows, nsteps_max);
   memcpy(out, s->series_host.data(), sizeof(float) * (size_t)r * np);
   return (int)r;
+}
+
+int dtm_probe_last(dtm_solver* s, float* out) {
+  if (!s || !out) return fail(DTM_ERR_ARG, "null argument");
+  const int np = (int)s->probes.size();
+  if (np == 0 || s->step == 0) return 0;
+  if (s->series_row > 0) {  // the newest row is still in the device buffer
+    HIPCHK(hipMemcpyAsync(out, s->series_dev + (size_t)(s->series_row - 1) * np, sizeof(float) * np,
+                          hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  } else {
+    if (s->series_host.size() < (size_t)np) return 0;
+    memcpy(out, s->series_host.data() + s->series_host.size() - np, sizeof(float) * np);
+  }
+  return 1;
 }
 
 int dtm_dft_read(dtm_solver* s, double* out) {
@@ -1160,6 +1256,19 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
   if (s->ade.on) return fail(DTM_ERR_ARG, "the adjoint does not cover dispersive or gyrotropic cells");
   if (!s->mt.d_on) return fail(DTM_ERR_STATE, "no design box (dtm_set_design_eps)");
   if (s->step != 0) return fail(DTM_ERR_STATE, "the adjoint starts at step 0");
+  if (s->pm.split) return fail(DTM_ERR_ARG, "the adjoint runs with the CPML, not the split-field layer");
+  // The backward pass steps the adjoint fields with the forward CPML update,
+  // which is the exact transpose outside the absorbing layer only: design
+  // cells and monitors must lie outside every slab (adjacent is exact).
+  auto in_layer = [&](int i, int j, int k) {
+    const int v[3] = {i, j, k}, n[3] = {c.nx, c.ny, c.nz};
+    for (int a = 0; a < 3; a++)
+      if (v[a] < c.npml[2 * a] || v[a] >= n[a] - c.npml[2 * a + 1]) return true;
+    return false;
+  };
+  if (in_layer(s->mt.d_i0, s->mt.d_j0, s->mt.d_k0) ||
+      in_layer(s->mt.d_i0 + s->mt.d_ni - 1, s->mt.d_j0 + s->mt.d_nj - 1, s->mt.d_k0 + s->mt.d_nk - 1))
+    return fail(DTM_ERR_ARG, "the design box reaches into the CPML (the adjoint is exact outside the layer only)");
   if (int rc = start(s)) return rc;
   const Materials& mt = s->mt;
   const long long nd = (long long)mt.d_ni * mt.d_nj * mt.d_nk;
@@ -1169,6 +1278,8 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
   for (int m = 0; m < nmon; m++) {
     const int i = mon_ijk[3 * m], j = mon_ijk[3 * m + 1], k = mon_ijk[3 * m + 2];
     if (i < 0 || i >= c.nx || j < 0 || j >= c.ny || k < 0 || k >= c.nz) return fail(DTM_ERR_ARG, "monitor outside grid");
+    if (in_layer(i, j, k))
+      return fail(DTM_ERR_ARG, "a monitor lies in the CPML (the adjoint is exact outside the layer only)");
     q[m] = ((long long)i * c.ny + j) * c.nz + k;
     ie[m] = s->classes[s->cls_host[q[m]]].inv_eps[2];
     const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
@@ -1236,6 +1347,7 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
       launch_adj_monitor(fields_ptr(s), hq, nmon, ser + (size_t)n * nmon, s->stream);
       if (n >= n_last) launch_adj_snap(s->g, fields_ptr(s), mt, hist + (size_t)(n - n_last + 1) * snap, s->stream);
     }
+    if (!rc) rc = launch_result();
     if (rc) break;
     std::vector<float> hs((size_t)nmon * nsteps);
     if (hipMemcpyAsync(hs.data(), ser, sizeof(float) * hs.size(), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
@@ -1306,6 +1418,7 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     }
     s->adjoint = false;
     s->replay = false;
+    if (!rc) rc = launch_result();
     if (rc) break;
     if (hipMemcpyAsync(grad_inv_eps, acc, sizeof(double) * nd, hipMemcpyDeviceToHost, s->stream) != hipSuccess) {
       rc = fail(DTM_ERR_DEVICE, "adjoint gradient copy");
@@ -1333,10 +1446,162 @@ int dtm_field_read(dtm_solver* s, int comp, float* buf) {
       if (int rc = dalloc(s, &s->tmp_dev, sizeof(float) * n)) return rc;
     const float inv = comp < 3 ? s->sc.ie[comp] : s->sc.ih[comp - 3];
     launch_to_float(1, comp_ptr(s, comp), inv, s->tmp_dev, (long long)n, s->stream);
+    if (int rc = launch_result()) return rc;
     HIPCHK(hipMemcpyAsync(buf, s->tmp_dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
+}
+
+// ------------------------------------------------------ save / resume
+// The time-stepped state in a fixed order: the current E and H, the current
+// CPML psi, the split-field parts and the ADE state (E^{n-1}, J per pole,
+// plasma current).  The fused step's second buffers are not state: every
+// step overwrites them completely.
+static std::vector<std::pair<void*, size_t>> canonical_state(const dtm_solver* s) {
+  std::unordered_map<const void*, size_t> sz;
+  for (auto& b : s->state) sz[b.first] = b.second;
+  std::vector<std::pair<void*, size_t>> out;
+  auto add = [&](void* p) {
+    auto it = sz.find(p);
+    if (p && it != sz.end()) out.emplace_back(p, it->second);
+  };
+  for (int c = 0; c < 3; c++) add(s->E[c]);
+  for (int c = 0; c < 3; c++) add(s->H[c]);
+  for (int a = 0; a < 3; a++)
+    for (int sl = 0; sl < 2; sl++) {
+      add(s->pm.psiE[a][sl]);
+      add(s->pm.psiH[a][sl]);
+    }
+  for (int c = 0; c < 3; c++) {
+    add(s->pm.p2E[c]);
+    add(s->pm.p2H[c]);
+  }
+  for (int c = 0; c < 3; c++) {
+    add(s->ade.Em[c]);
+    for (int p = 0; p < kMaxPoles; p++) {
+      add(s->ade.Jp[c][p]);
+      add(s->ade.Jm[c][p]);
+    }
+    add(s->ade.Jg[c]);
+  }
+  return out;
+}
+
+namespace {
+struct StateHeader {
+  char magic[8];  // "DTMSTAT1"
+  int nx, ny, nz, storage, pml_kind, nbuf;
+  long long step;
+  long long dft_values, flux_values;
+};
+}  // namespace
+
+int dtm_save_state(dtm_solver* s, const char* path) {
+  if (!s || !path) return fail(DTM_ERR_ARG, "null argument");
+  if (int rc = start(s)) return rc;
+  const auto st = canonical_state(s);
+  StateHeader h{};
+  memcpy(h.magic, "DTMSTAT1", 8);
+  h.nx = s->cfg.nx;
+  h.ny = s->cfg.ny;
+  h.nz = s->cfg.nz;
+  h.storage = s->store;
+  h.pml_kind = s->cfg.pml_kind;
+  h.nbuf = (int)st.size();
+  h.step = s->step;
+  h.dft_values = s->dft_dev ? 2 * (long long)s->probes.size() * s->nfreq : 0;
+  h.flux_values = s->fs.total ? s->fs.nface : 0;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(DTM_ERR_ARG, std::string("cannot open ") + path + " for writing");
+  bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+  std::vector<char> buf;
+  for (auto& b : st) {
+    const unsigned long long n = b.second;
+    buf.resize(b.second);
+    if (hipMemcpyAsync(buf.data(), b.first, b.second, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
+      fclose(f);
+      return fail(DTM_ERR_DEVICE, "state read-back");
+    }
+    ok = ok && fwrite(&n, sizeof(n), 1, f) == 1 && fwrite(buf.data(), 1, b.second, f) == b.second;
+  }
+  auto dump = [&](const double* dev, long long cnt) {
+    if (cnt <= 0) return;
+    std::vector<double> v((size_t)cnt);
+    if (hipMemcpy(v.data(), dev, sizeof(double) * (size_t)cnt, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    ok = ok && fwrite(v.data(), sizeof(double), (size_t)cnt, f) == (size_t)cnt;
+  };
+  dump(s->dft_dev, h.dft_values);
+  dump(s->fs.total, h.flux_values);
+  if (fclose(f) != 0) ok = false;
+  return ok ? 0 : fail(DTM_ERR_DEVICE, std::string("writing ") + path + " failed");
+}
+
+int dtm_load_state(dtm_solver* s, const char* path) {
+  if (!s || !path) return fail(DTM_ERR_ARG, "null argument");
+  if (int rc = start(s)) return rc;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(DTM_ERR_ARG, std::string("cannot open ") + path);
+  StateHeader h{};
+  const auto st = canonical_state(s);
+  const long long ndft = s->dft_dev ? 2 * (long long)s->probes.size() * s->nfreq : 0;
+  const long long nflux = s->fs.total ? s->fs.nface : 0;
+  if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, "DTMSTAT1", 8) != 0) {
+    fclose(f);
+    return fail(DTM_ERR_ARG, std::string(path) + " is not a saved solver state");
+  }
+  if (h.nx != s->cfg.nx || h.ny != s->cfg.ny || h.nz != s->cfg.nz || h.storage != s->store ||
+      h.pml_kind != s->cfg.pml_kind || h.nbuf != (int)st.size() || h.dft_values != ndft || h.flux_values != nflux) {
+    fclose(f);
+    return fail(DTM_ERR_ARG, "saved state does not match this solver's grid, storage, layer, materials, probes "
+                             "or flux boxes");
+  }
+  std::vector<char> buf;
+  for (auto& b : st) {
+    unsigned long long n = 0;
+    buf.resize(b.second);
+    if (fread(&n, sizeof(n), 1, f) != 1 || n != b.second || fread(buf.data(), 1, b.second, f) != b.second) {
+      fclose(f);
+      return fail(DTM_ERR_ARG, "saved state buffer sizes do not match this solver");
+    }
+    if (hipMemcpy(b.first, buf.data(), b.second, hipMemcpyHostToDevice) != hipSuccess) {
+      fclose(f);
+      return fail(DTM_ERR_DEVICE, "state upload");
+    }
+  }
+  auto restore = [&](double* dev, long long cnt) -> bool {
+    if (cnt <= 0) return true;
+    std::vector<double> v((size_t)cnt);
+    return fread(v.data(), sizeof(double), (size_t)cnt, f) == (size_t)cnt &&
+           hipMemcpy(dev, v.data(), sizeof(double) * (size_t)cnt, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  const bool ok = restore(s->dft_dev, ndft) && restore(s->fs.total, nflux);
+  fclose(f);
+  if (!ok) return fail(DTM_ERR_ARG, "saved state is truncated");
+  if (int rc = flush_series(s)) return rc;
+  s->series_host.clear();
+  s->flux_host.clear();
+  s->step = h.step;
+  s->tab_n0 = -1;
+  return 0;
+}
+
+// One component as an aivlib .arr file (the layout SeismoDrops::drop writes,
+// drop.cu:84-107): int 12, three zero ints (a 12-byte comment), int dim = 3,
+// int sizeof(float), the extents fastest first (nz, ny, nx), then the values
+// with z fastest.
+int dtm_write_arr(dtm_solver* s, int comp, const char* path) {
+  if (!s || !path || comp < 0 || comp > 5) return fail(DTM_ERR_ARG, "bad .arr arguments");
+  const size_t n = (size_t)s->cfg.nx * s->cfg.ny * s->cfg.nz;
+  std::vector<float> v(n);
+  if (int rc = dtm_field_read(s, comp, v.data())) return rc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(DTM_ERR_ARG, std::string("cannot open ") + path + " for writing");
+  const int head[9] = {12, 0, 0, 0, 3, (int)sizeof(float), s->cfg.nz, s->cfg.ny, s->cfg.nx};
+  bool ok = fwrite(head, sizeof(int), 9, f) == 9 && fwrite(v.data(), sizeof(float), n, f) == n;
+  if (fclose(f) != 0) ok = false;
+  return ok ? 0 : fail(DTM_ERR_DEVICE, std::string("writing ") + path + " failed");
 }
 
 int dtm_field_write(dtm_solver* s, int comp, const float* buf) {
@@ -1350,19 +1615,96 @@ int dtm_field_write(dtm_solver* s, int comp, const float* buf) {
     HIPCHK(hipMemcpyAsync(s->tmp_dev, buf, sizeof(float) * n, hipMemcpyHostToDevice, s->stream));
     const float sc = comp < 3 ? s->sc.se[comp] : s->sc.sh[comp - 3];
     launch_from_float(1, s->tmp_dev, sc, comp_ptr(s, comp), (long long)n, s->stream);
+    if (int rc = launch_result()) return rc;
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 
-int dtm_energy(dtm_solver* s, double* out) {
+static int box_ok(const dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1) {
+  const dtm_config& c = s->cfg;
+  return i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 < i1 && j0 < j1 && k0 < k1 && i1 <= c.nx && j1 <= c.ny && k1 <= c.nz;
+}
+
+int dtm_energy_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1, double* out) {
   if (!s || !out) return fail(DTM_ERR_ARG, "null argument");
-  HIPCHK(hipMemsetAsync(s->energy_dev, 0, sizeof(double), s->stream));
-  launch_energy(s->store, s->g, fields_ptr(s), s->sc, s->mt, s->energy_dev, s->stream);
-  double v = 0;
-  HIPCHK(hipMemcpyAsync(&v, s->energy_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  if (!box_ok(s, i0, i1, j0, j1, k0, k1)) return fail(DTM_ERR_ARG, "energy box outside the grid or empty");
+  const int ni = i1 - i0;
+  if (ni > s->energy_part_cap) {  // one FP64 partial per x plane of the box
+    if (int rc = dalloc(s, &s->energy_part, sizeof(double) * (size_t)ni)) return rc;
+    s->energy_part_cap = ni;
+  }
+  const dtm_config& c = s->cfg;
+  launch_energy_box(s->store, s->g, fields_ptr(s), s->sc, s->mt, Region{i0, i1, j0, j1, k0, k1}, s->energy_part,
+                    c.dx * c.dy * c.dz, s->energy_dev, s->stream);
+  if (int rc = launch_result()) return rc;
+  HIPCHK(hipMemcpyAsync(out, s->energy_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  *out = v * s->cfg.dx * s->cfg.dy * s->cfg.dz;
+  return 0;
+}
+
+int dtm_energy(dtm_solver* s, double* out) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  return dtm_energy_box(s, 0, s->cfg.nx, 0, s->cfg.ny, 0, s->cfg.nz, out);
+}
+
+int dtm_add_flux_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  if (s->started) return fail(DTM_ERR_STATE, "flux boxes must be added before the first step");
+  if ((int)s->flux_faces.size() >= 6 * kMaxFluxBoxes) return fail(DTM_ERR_LIMIT, "too many flux boxes");
+  const dtm_config& c = s->cfg;
+  const int lo[3] = {i0, j0, k0}, hi[3] = {i1, j1, k1}, n[3] = {c.nx, c.ny, c.nz};
+  const double d[3] = {c.dx, c.dy, c.dz};
+  // every face plane needs the E cells on both sides and the H cell on it
+  for (int a = 0; a < 3; a++)
+    if (lo[a] < 1 || hi[a] <= lo[a] || hi[a] > n[a] - 1)
+      return fail(DTM_ERR_ARG, "flux box faces must lie on planes 1 .. n-1 of every axis, low < high");
+  const int id = (int)s->flux_faces.size() / 6;
+  for (int a = 0; a < 3; a++) {
+    const int b = (a + 1) % 3, e = (a + 2) % 3;
+    for (int side = 0; side < 2; side++) {
+      FluxFace f{};
+      f.a = a;
+      f.pa = side ? hi[a] : lo[a];
+      f.b0 = lo[b];
+      f.b1 = hi[b];
+      f.c0 = lo[e];
+      f.c1 = hi[e];
+      const long long nb = hi[b] - lo[b], nc = hi[e] - lo[e];
+      const long long nodes = (nb + 1) * nc + nb * (nc + 1);
+      if (nodes + s->flux_snap_n > 0x7fffffffLL) return fail(DTM_ERR_LIMIT, "flux faces too large");
+      f.nodes = (int)nodes;
+      f.snap0 = s->flux_snap_n;
+      s->flux_snap_n += (int)nodes;
+      f.area = d[b] * d[e];
+      f.sign = side ? 1.0 : -1.0;
+      s->flux_faces.push_back(f);
+    }
+  }
+  return id;
+}
+
+int dtm_flux_read(dtm_solver* s, double* out, int nsteps_max) {
+  if (!s || (!out && nsteps_max > 0)) return fail(DTM_ERR_ARG, "null argument");
+  if (int rc = flush_series(s)) return rc;
+  const int nf = (int)s->flux_faces.size();
+  if (nf == 0) return 0;
+  const long long rows = (long long)(s->flux_host.size() / nf);
+  const long long r = std::min<long long>(rows, nsteps_max);
+  memcpy(out, s->flux_host.data(), sizeof(double) * (size_t)r * nf);
+  return (int)r;
+}
+
+int dtm_flux_total(dtm_solver* s, double* out) {
+  if (!s || !out) return fail(DTM_ERR_ARG, "null argument");
+  const int nf = (int)s->flux_faces.size();
+  if (nf == 0) return 0;
+  if (!s->fs.total) {
+    memset(out, 0, sizeof(double) * nf);
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(out, s->fs.total, sizeof(double) * nf, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 

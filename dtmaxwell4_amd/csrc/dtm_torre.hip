@@ -479,10 +479,7 @@ __global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Scales sc, const uint8
 }
 
 // ------------------------------------------------------------ launcher
-static void check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) fprintf(stderr, "dtm: launch of %s failed: %s\n", what, hipGetErrorString(e));
-}
+static void check_launch(const char* what) { note_launch(what); }
 
 TorreSet make_torre_set(const Region* regs, int nreg, int seg) {
   TorreSet s{};

@@ -136,16 +136,61 @@ class Solver:
         r = abi.check(self.L.dtm_probe_read(self.h, _ptr(out), rows), "dtm_probe_read")
         return out[:r, :self.nprobe]
 
+    def probe_last(self):
+        """Probe values after the last step (None before the first step)."""
+        out = np.zeros(max(self.nprobe, 1), dtype=np.float32)
+        r = abi.check(self.L.dtm_probe_last(self.h, _ptr(out)), "dtm_probe_last")
+        return out[:self.nprobe] if r > 0 else None
+
     def dft(self):
         out = np.zeros((max(self.nprobe, 1), max(self.nfreq, 1), 2), dtype=np.float64)
         abi.check(self.L.dtm_dft_read(self.h, _ptr(out)), "dtm_dft_read")
         out = out[:self.nprobe, :self.nfreq]
         return out[..., 0] + 1j * out[..., 1]
 
-    def energy(self):
+    def energy(self, box=None):
+        """1/2 sum (eps E^2 + H^2) dV in FP64 over the grid, or over the cells
+        box = (i0, i1, j0, j1, k0, k1)."""
         v = C.c_double()
-        abi.check(self.L.dtm_energy(self.h, C.byref(v)), "dtm_energy")
+        if box is None:
+            abi.check(self.L.dtm_energy(self.h, C.byref(v)), "dtm_energy")
+        else:
+            abi.check(self.L.dtm_energy_box(self.h, *[int(b) for b in box], C.byref(v)), "dtm_energy_box")
         return v.value
+
+    def add_flux_box(self, i0, i1, j0, j1, k0, k1):
+        """Record the FP64 Poynting flux through the six faces of a box every
+        step; returns the box id."""
+        r = abi.check(self.L.dtm_add_flux_box(self.h, i0, i1, j0, j1, k0, k1), "dtm_add_flux_box")
+        self.nflux = getattr(self, "nflux", 0) + 1
+        return r
+
+    def flux(self):
+        """Flux series [step][box][face] (faces x-, x+, y-, y+, z-, z+, outward)."""
+        nb = getattr(self, "nflux", 0)
+        rows = self.time_step
+        out = np.zeros((max(rows, 1), max(6 * nb, 1)), dtype=np.float64)
+        r = abi.check(self.L.dtm_flux_read(self.h, _ptr(out), rows), "dtm_flux_read")
+        return out[:r, :6 * nb].reshape(r, nb, 6)
+
+    def flux_total(self):
+        """Running FP64 sums of flux * dt: [box][face]."""
+        nb = getattr(self, "nflux", 0)
+        out = np.zeros(max(6 * nb, 1), dtype=np.float64)
+        abi.check(self.L.dtm_flux_total(self.h, _ptr(out)), "dtm_flux_total")
+        return out[:6 * nb].reshape(nb, 6)
+
+    def write_arr(self, comp, path):
+        """One component as an aivlib .arr snapshot (drop.cu:84-107 layout)."""
+        abi.check(self.L.dtm_write_arr(self.h, comp, str(path).encode()), "dtm_write_arr")
+
+    def save_state(self, path):
+        """The whole time-stepped state, step count, DFT bins and flux totals."""
+        abi.check(self.L.dtm_save_state(self.h, str(path).encode()), "dtm_save_state")
+
+    def load_state(self, path):
+        """Resume from dtm_save_state (same config, materials, probes, boxes)."""
+        abi.check(self.L.dtm_load_state(self.h, str(path).encode()), "dtm_load_state")
 
     def set_adjoint_checkpoints(self, every):
         """Adjoint memory scheme: 0 keeps the design-box E of every step;

@@ -226,14 +226,54 @@ int dtm_sync(dtm_solver* s);
 /* Probe/sensor series: out[step][probe], nsteps_max rows; returns rows. */
 int dtm_probe_count(const dtm_solver* s);
 int dtm_probe_read(dtm_solver* s, float* out, int nsteps_max);
+/* Newest row of the probe series (the values after the last step), one
+ * float per probe: what Sensor::write reads once per calcStep
+ * (sensor.h:51-83, diamond.cu:473-475).  Returns 1, or 0 before the first
+ * step or without probes. */
+int dtm_probe_last(dtm_solver* s, float* out);
 /* DFT bins: out[probe][freq][re,im] doubles. */
 int dtm_dft_read(dtm_solver* s, double* out);
 
 /* Field access: comp DTM_EX..DTM_HZ, buf holds nx*ny*nz floats. */
 int dtm_field_read(dtm_solver* s, int comp, float* buf);
 int dtm_field_write(dtm_solver* s, int comp, const float* buf);
-/* Discrete field energy sum(eps E^2 + H^2)*dV (double). */
+/* One component written as an aivlib .arr file, the snapshot format of the
+ * reference's SeismoDrops::drop (drop.cu:84-107): int 12, three zero ints,
+ * int 3 (dimensions), int 4 (bytes per value), the extents nz, ny, nx, then
+ * nx*ny*nz floats with z fastest. */
+int dtm_write_arr(dtm_solver* s, int comp, const char* path);
+/* Save / resume: the whole time-stepped state (current fields, CPML psi,
+ * split-field parts, dispersive and plasma currents), the step count, the
+ * FP64 DFT bins and flux totals, to a file; dtm_load_state restores it into
+ * a solver created with the same config, materials, probes, DFT and flux
+ * boxes, which then continues bit for bit as the saved one would have.  The
+ * probe and flux series recorded before the save are not in the file.
+ * Replaces the reference's --swap_dir window swap files (window.hpp:177-267)
+ * as the restart path; no file format of the reference is kept. */
+int dtm_save_state(dtm_solver* s, const char* path);
+int dtm_load_state(dtm_solver* s, const char* path);
+/* Field energy 1/2 sum(eps_inf E^2 + H^2) dV of the stored state (E^n and
+ * H^{n-1/2}), summed in FP64 on the device in a fixed order (the same value
+ * on every call).  dtm_energy: the whole grid; dtm_energy_box: the cells
+ * [i0,i1) x [j0,j1) x [k0,k1).  The design box's eps is used where it is set.
+ * No counterpart in the reference. */
 int dtm_energy(dtm_solver* s, double* out);
+int dtm_energy_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1, double* out);
+
+/* ------------------------------------------------------- Poynting flux */
+/* FP64 Poynting-flux sums through the six faces of a box of cells
+ * [i0,i1) x [j0,j1) x [k0,k1), recorded on the device every step: the
+ * outward flux of S = E x H through the faces x-, x+, y-, y+, z-, z+ at
+ * t = (n+1/2) dt, from H^{n+1/2} and (E^n + E^{n+1})/2 averaged onto the face
+ * planes (second order in space and time).  Every face plane must lie on
+ * planes 1 .. n-1 of its axis.  Up to 8 boxes, added before the first step;
+ * returns the box id.  No counterpart in the reference (it writes point
+ * sensors only, sensor.h). */
+int dtm_add_flux_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1);
+/* Flux series: out[step][box*6 + face] doubles, nsteps_max rows; returns rows. */
+int dtm_flux_read(dtm_solver* s, double* out, int nsteps_max);
+/* Running FP64 time integrals sum_n flux dt: out[box*6 + face]. */
+int dtm_flux_total(dtm_solver* s, double* out);
 
 /* -------------------------------------------------------------- adjoint */
 /* Gradient of J = sum_m |F_m|^2, F_m = sum_{n<nsteps} Ez_m^{n+1} e^{i omega (n+1) dt} dt,

@@ -22,6 +22,22 @@ def small_case(npml=6, steps=260, seed=3):
     return cfg, d0, eps, mons, 2 * np.pi / 0.5, steps
 
 
+def downsized_case(steps=800):
+    """The inverse-design workload downsized to 48^3: CPML 8, design box
+    16x16x8 of rho ~ U[0,1] (seeded, 3-cell box smoothing), eps = 1 + 11 rho,
+    soft Ez dipole 4 cells before the box, 4x4 monitor plane 4 cells behind
+    it, one frequency; the Gaussian pulse is shortened (width 1.5 periods,
+    delay 6) so that 800 steps carry it through the box."""
+    w = configs.inverse_design(nx=48, ny=48, nz=48, npml=8, design=(16, 16, 8), steps=steps, monitor=4)
+    so = w.cfg.soft[0]
+    f0 = w.notes["f0"]
+    so.tw, so.t0 = 1.5 / f0, 6.0 / f0
+    d0 = w.design[:3]
+    eps = w.design[3]
+    mons = [p[1:] for p in w.probes]
+    return w.cfg, d0, eps, mons, float(w.omegas[0]), steps, w.notes["rho"]
+
+
 def objective(cfg, d0, eps, mons, omega, steps):
     """J = sum_m |sum_n Ez_m^{n+1} e^{i w (n+1) dt} dt|^2 from the FP64 oracle."""
     o = Oracle(cfg)

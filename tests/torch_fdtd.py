@@ -53,14 +53,15 @@ def d_bwd(f, axis, d):
     return (27 * (f - _shift(f, axis, -1)) - (_shift(f, axis, 1) - _shift(f, axis, -2))) / (24 * d)
 
 
-def run(cfg, eps_design, d0, monitors, omega, steps):
+def run(cfg, eps_design, d0, monitors, omega, steps, device=None):
     """J = sum_m |sum_n Ez_m^{n+1} e^{i w (n+1) dt} dt|^2; eps_design is a
-    torch float64 tensor (it may require grad)."""
+    torch float64 tensor (it may require grad).  `device` places the unrolled
+    run (default: the CPU)."""
     dt = cfg.dt
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     n3 = (nx, ny, nz)
     dd = (cfg.dx, cfg.dy, cfg.dz)
-    kw = dict(dtype=torch.float64)
+    kw = dict(dtype=torch.float64, device=device)
     # CPML b, c along each axis at E (odd) and H (even) half-coordinates
     bc = {}
     for a in range(3):
@@ -75,14 +76,14 @@ def run(cfg, eps_design, d0, monitors, omega, steps):
     i0, j0, k0 = d0
     ni, nj, nk = eps_design.shape
     inv_eps = inv_eps.clone()
-    inv_eps[i0:i0 + ni, j0:j0 + nj, k0:k0 + nk] = 1.0 / eps_design
+    inv_eps[i0:i0 + ni, j0:j0 + nj, k0:k0 + nk] = 1.0 / eps_design.to(inv_eps.device)
     E = [torch.zeros(n3, **kw) for _ in range(3)]
     H = [torch.zeros(n3, **kw) for _ in range(3)]
     # curl terms per component: (axis, field component, sign)
     curl = {0: ((1, 2, 1), (2, 1, -1)), 1: ((2, 0, 1), (0, 2, -1)), 2: ((0, 1, 1), (1, 0, -1))}
     psiH = {(c, a): torch.zeros(n3, **kw) for c in range(3) for (a, _, _) in curl[c]}
     psiE = {(c, a): torch.zeros(n3, **kw) for c in range(3) for (a, _, _) in curl[c]}
-    mon = torch.tensor(np.array(monitors), dtype=torch.long)
+    mon = torch.tensor(np.array(monitors), dtype=torch.long, device=device)
     Fr = torch.zeros(len(monitors), **kw)
     Fi = torch.zeros(len(monitors), **kw)
     for n in range(steps):
