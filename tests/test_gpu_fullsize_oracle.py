@@ -142,9 +142,11 @@ def test_gyro_slab_full_size_8000_steps():
     # step 4000 hold the FP32 bound.  By step 8000 the pulse has left and the
     # remaining field is 1/15 of its step-4000 amplitude: the FP32 CPU
     # restatement of the same equations (liboracle32) itself sits at 3.0e-5
-    # from FP64 there (GPU: 1.7e-5), so the end-state plane is held to that.
-    assert max(v for k, v in e.items() if k != "plane_8000") <= TOL
-    assert e["plane_8000"] <= 3e-5
+    # from FP64 there (GPU: 1.7e-5), so the end-state plane and the end-state
+    # sums of squares (GPU: 2.3e-5) are held to that.
+    end = ("plane_8000", "sumsq_8000")
+    assert max(v for k, v in e.items() if k not in end) <= TOL
+    assert max(e[k] for k in end) <= 3e-5
     # the full 256x256x512 grid carries the same fields: equal probe series
     _, full = gpu_run("gyro_thin", nx=256, ny=256)
     d = np.abs(full["probes"] - thin["probes"]).max() / np.abs(thin["probes"]).max()
