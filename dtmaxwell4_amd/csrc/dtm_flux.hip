@@ -24,6 +24,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "dtm_internal.h"
 
 namespace dtm {
@@ -164,6 +166,47 @@ __global__ __launch_bounds__(256) void k_energy_final(const double* part, int n,
 }
 
 }  // namespace
+
+// ------------------------------------------- box energy without allocation
+// Scratch of dtm_energy_box_noalloc: the plane partials of one box (at most
+// kStaticPlanes x planes) and the result in the last slot.  Every device the
+// library runs on holds its own copy until the process ends; calls that
+// overlap on one device share it and race.  Nothing in it must start at
+// zero: each call overwrites the partials it sums and the result slot.
+constexpr int kStaticPlanes = 8192;
+static __device__ __align__(256) double scratch[kStaticPlanes + 1];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int static_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != dev) return -1;  // the device must be current
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return -1;
+  g_scratch_ptr[dev] = static_cast<double*>(p);  // this device's copy; host code never uses `scratch` itself
+  return 0;
+}
+
+int static_scratch_planes() { return kStaticPlanes; }
+
+int energy_box_static(int dev, int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt,
+                      const Region& bx, double dv, double* host_out, hipStream_t st) {
+  if (bx.i1 - bx.i0 > kStaticPlanes) return -2;
+  if (static_scratch_init(dev)) return -1;
+  double* ptr = g_scratch_ptr[dev];
+  launch_energy_box(store, g, fields, sc, mt, bx, ptr, dv, ptr + kStaticPlanes, st);
+  if (hipMemcpyAsync(host_out, ptr + kStaticPlanes, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+    return -1;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : -1;
+}
 
 void launch_flux_snap(int store, const Grid& g, const void* fields, const Scales& sc, const FluxSet& fs,
                       hipStream_t st) {

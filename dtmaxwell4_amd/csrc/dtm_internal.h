@@ -278,6 +278,15 @@ void launch_flux(int store, const Grid& g, const void* fields, const Scales& sc,
 // the box in part[0 .. i1-i0), then their fixed-order sum into *out.
 void launch_energy_box(int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt,
                        const Region& bx, double* part, double dv, double* out, hipStream_t st);
+// The same box energy with the partials in a static device array instead of
+// allocated memory (dtm_flux.hip).  static_scratch_init looks the array up on
+// device `dev`, which must be current (0, or -1 on failure);
+// energy_box_static returns 0, -1 (device error) or -2 (more x planes than
+// static_scratch_planes()).
+int static_scratch_init(int dev);
+int static_scratch_planes();
+int energy_box_static(int dev, int store, const Grid& g, const void* fields, const Scales& sc, const Materials& mt,
+                      const Region& bx, double dv, double* host_out, hipStream_t st);
 void launch_to_float(int store, const void* src, float scale_inv, float* dst, long long n,
                      hipStream_t st);
 void launch_from_float(int store, const float* src, float scale, void* dst, long long n,

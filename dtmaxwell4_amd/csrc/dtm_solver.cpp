@@ -804,6 +804,11 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
     delete s;
     return fail(DTM_ERR_DEVICE, hipGetErrorString(e));
   }
+  if (static_scratch_init(s->dev)) {  // device address of dtm_energy_box_noalloc's static scratch
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+    return fail(DTM_ERR_DEVICE, "hipGetSymbolAddress of the static scratch failed");
+  }
   Grid& g = s->g;
   g.nx = c.nx;
   g.ny = c.ny;
@@ -1640,6 +1645,20 @@ int dtm_energy_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1
   if (int rc = launch_result()) return rc;
   HIPCHK(hipMemcpyAsync(out, s->energy_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int dtm_energy_box_noalloc(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1, double* out) {
+  if (!s || !out) return fail(DTM_ERR_ARG, "null argument");
+  if (!box_ok(s, i0, i1, j0, j1, k0, k1)) return fail(DTM_ERR_ARG, "energy box outside the grid or empty");
+  const dtm_config& c = s->cfg;
+  const int rc = energy_box_static(s->dev, s->store, s->g, fields_ptr(s), s->sc, s->mt, Region{i0, i1, j0, j1, k0, k1},
+                                   c.dx * c.dy * c.dz, out, s->stream);
+  if (rc == -2)
+    return fail(DTM_ERR_LIMIT, "dtm_energy_box_noalloc: the box spans more than " +
+                                   std::to_string(static_scratch_planes()) + " x planes (its static scratch)");
+  if (int lr = launch_result()) return lr;
+  if (rc) return fail(DTM_ERR_DEVICE, "dtm_energy_box_noalloc: device error");
   return 0;
 }
 

@@ -259,6 +259,16 @@ int dtm_load_state(dtm_solver* s, const char* path);
  * No counterpart in the reference. */
 int dtm_energy(dtm_solver* s, double* out);
 int dtm_energy_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1, double* out);
+/* dtm_energy_box for callers that cannot allocate device memory: the same
+ * kernels and the same value, with the per-plane partials in a static device
+ * array of the library (8192 doubles plus the result, held by every device
+ * the library runs on until the process ends) instead of memory the library
+ * allocates on first use.  Calls no hipMalloc or hipFree; boxes spanning
+ * more than 8192 x planes get DTM_ERR_LIMIT.  Every call on a device shares
+ * that device's array: calls that overlap on one device race.  No
+ * counterpart in the reference; for callers that cannot allocate (code
+ * under stream capture, tight serving loops). */
+int dtm_energy_box_noalloc(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1, double* out);
 
 /* ------------------------------------------------------- Poynting flux */
 /* FP64 Poynting-flux sums through the six faces of a box of cells
