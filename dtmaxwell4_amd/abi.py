@@ -14,7 +14,7 @@ STORE_FP32, STORE_FP16 = 0, 1
 SWEEP_TWOPASS, SWEEP_FUSED = 0, 1
 PML_CPML, PML_SPLIT = 0, 1
 TFSF_NONE, TFSF_DIPOLE, TFSF_PLANE = 0, 1, 2
-WAVE_RAISED_COS, WAVE_GAUSS_SINE = 0, 1
+WAVE_RAISED_COS, WAVE_GAUSS_SINE, WAVE_GAUSS = 0, 1, 2
 MAX_MATS, MAX_POLES, MAX_SOFT = 64, 4, 16
 
 ERRORS = {

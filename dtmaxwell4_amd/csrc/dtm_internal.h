@@ -152,6 +152,10 @@ struct TorrePml {
 struct TorreCold {
   const void* fo[6];
   void* fn[6];
+  // the same arrays as one base each: component c at byte c * cbytes
+  const char* fob;
+  char* fnb;
+  unsigned cbytes;
   TorrePml zp;
   Cpml pmo, pmn;  // old / new psi (x and y slabs)
   Materials mt;

@@ -73,12 +73,18 @@ __device__ __forceinline__ void wave3(int waveform, double p, double w, double k
     f0 = sp * 0.5 * (1.0 - ck);
     f1 = 0.5 * cp * (1.0 - ck) + 0.5 * kEnv * sp * sk;
     f2 = -0.5 * sp * (1.0 - ck) + kEnv * cp * sk + 0.5 * kEnv * kEnv * sp * ck;
-  } else {  // Gaussian-modulated sine
+  } else {  // Gaussian-modulated sine, or the bare Gaussian (waveform 2, signal.hpp:55-57)
     const double a = 1.0 / (w * tw);
     const double u = (p / w - t0) / tw;
     const double g = exp(-u * u);
     const double g1 = -2.0 * u * a * g;
     const double g2 = (4.0 * u * u * a * a - 2.0 * a * a) * g;
+    if (waveform == 2) {
+      f0 = g;
+      f1 = g1;
+      f2 = g2;
+      return;
+    }
     double sp, cp;
     sincos(p, &sp, &cp);
     f0 = g * sp;

@@ -7,7 +7,9 @@
 //    GridNx/GridNy/GridNz are the true x/y/z extents (the reference's
 //    Python-visible GridNy/GridNz are swapped, py_consts.h:10-12 vs
 //    params.h:44-49);
-//  * the absorbing layer is CPML on all six faces, Npml* cells per side;
+//  * the absorbing layer is CPML, Npml* cells per side, on every face but
+//    the low-z one (non-absorbing there, as in the reference, DTmxw.cu:262;
+//    the low-z face is a zero boundary inside the silver film);
 //  * no OpenGL viewer: without --test, _main reports that and returns -1.
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -261,7 +263,10 @@ int _main(int argc, char** argv) {
   c.dt = dt;
   c.npml[0] = c.npml[1] = Npmlx;
   c.npml[2] = c.npml[3] = Npmly;
-  c.npml[4] = c.npml[5] = Npmlz;
+  // no layer at the low-z face, as in the reference (k2z = 1 for the low
+  // half of its z table, DTmxw.cu:262): that face lies in the silver
+  c.npml[4] = 0;
+  c.npml[5] = Npmlz;
   c.pml_vmax = shotpoint.V_max > 0 ? shotpoint.V_max : 1.0;
   c.ntime = Ntime;
   set_materials(c);

@@ -315,6 +315,7 @@ static double wave_value(int waveform, double w, double kEnv, double t0, double 
     return sin(p) * 0.5 * (1 - cos(kp));
   }
   const double u = (p / w - t0) / tw;
+  if (waveform == DTM_WAVE_GAUSS) return exp(-u * u);
   return exp(-u * u) * sin(p);
 }
 
@@ -664,11 +665,22 @@ static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]) {
       any_special = true;
       if (s->classes[id].special & 8) any_gyro = true;
       const int i = (int)(q / ((size_t)c.ny * c.nz)), j = (int)((q / c.nz) % c.ny), k = (int)(q % c.nz);
-      if (s->pm.split) {
-        const int v[3] = {i, j, k}, nn[3] = {c.nx, c.ny, c.nz};
-        for (int a = 0; a < 3; a++)
-          if (v[a] < c.npml[2 * a] || v[a] >= nn[a] - c.npml[2 * a + 1])
-            return fail(DTM_ERR_ARG, "the split-field layer does not cover dispersive or gyrotropic cells");
+      const int v[3] = {i, j, k}, nn[3] = {c.nx, c.ny, c.nz};
+      bool in_layer = false;
+      for (int a = 0; a < 3; a++)
+        if (v[a] < c.npml[2 * a] || v[a] >= nn[a] - c.npml[2 * a + 1]) in_layer = true;
+      if (in_layer && s->pm.split)
+        return fail(DTM_ERR_ARG, "the split-field layer does not cover dispersive or gyrotropic cells");
+      // A Drude/Lorentz cell inside a CPML slab makes the CPML-ADE update grow
+      // without bound (a silver half-space in the layer: the oracle and the
+      // GPU both diverge after ~2000 steps); the reference keeps its layer out
+      // of the metal too (non-absorbing low-z face, DTmxw.cu:262).
+      if (in_layer) {
+        const auto& cm = s->class_mats[id];
+        for (int q3 = 0; q3 < 3; q3++)
+          if (c.mat[cm[q3]].npoles > 0)
+            return fail(DTM_ERR_ARG, "a dispersive (Drude/Lorentz) cell lies in a CPML slab: leave that face "
+                                     "without a layer (npml = 0) or keep the medium out of the layer");
       }
       bb[0] = std::min(bb[0], i);
       bb[1] = std::max(bb[1], i);
@@ -767,6 +779,10 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   if (c.pml_kind != DTM_PML_CPML && c.pml_kind != DTM_PML_SPLIT) return fail(DTM_ERR_ARG, "bad pml_kind");
   if (c.pml_kind == DTM_PML_SPLIT && c.sweep == DTM_SWEEP_FUSED)
     return fail(DTM_ERR_ARG, "the split-field layer runs with the two-pass sweeps");
+  if (c.sweep == DTM_SWEEP_FUSED &&
+      6ull * ((unsigned long long)c.nx * c.ny * c.nz + kPad) * (c.storage ? 2 : 4) >= (1ull << 32))
+    return fail(DTM_ERR_LIMIT, "the fused step addresses a field set (six arrays) with 32-bit byte offsets: "
+                               "at most 4 GiB, use the two-pass sweeps for larger grids");
   const int n3[3] = {c.nx, c.ny, c.nz};
   for (int a = 0; a < 3; a++) {
     if (c.npml[2 * a] < 0 || c.npml[2 * a + 1] < 0 || c.npml[2 * a] + c.npml[2 * a + 1] > n3[a])
@@ -838,13 +854,23 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   g.zidx = (unsigned)((size_t)c.nx * c.ny * c.nz);
   const size_t n = (size_t)c.nx * c.ny * c.nz;
   const size_t esz = s->store ? 2 : 4;
-  int rc = 0;
-  for (int q = 0; q < 3 && !rc; q++) {
-    rc = sdalloc(s, &s->E[q], esz * (n + kPad));
-    if (!rc) rc = sdalloc(s, &s->H[q], esz * (n + kPad));
-    if (!rc && s->torre) rc = sdalloc(s, &s->E2[q], esz * (n + kPad));
-    if (!rc && s->torre) rc = sdalloc(s, &s->H2[q], esz * (n + kPad));
-  }
+  // the six components of a field set share one allocation (Ex Ey Ez Hx Hy
+  // Hz, each n + kPad elements), so the fused step addresses all of them from
+  // one base with 32-bit byte offsets
+  auto field_set = [&](void* e[3], void* h[3]) -> int {
+    const size_t per = esz * (n + kPad);
+    char* b = nullptr;
+    if (int rc2 = dalloc(s, &b, 6 * per)) return rc2;
+    for (int q = 0; q < 3; q++) {
+      e[q] = b + q * per;
+      h[q] = b + (3 + q) * per;
+    }
+    for (int q = 0; q < 3; q++) s->state.emplace_back(e[q], per);
+    for (int q = 0; q < 3; q++) s->state.emplace_back(h[q], per);
+    return 0;
+  };
+  int rc = field_set(s->E, s->H);
+  if (!rc && s->torre) rc = field_set(s->E2, s->H2);
   if (!rc) rc = setup_cpml(s);
   if (!rc) rc = setup_tfsf(s);
   if (!rc) {

@@ -30,6 +30,8 @@ def waveform(kind, w, kEnv, t0, tw, t):
         on = (kp >= 0) & (kp <= 2 * np.pi)
         return np.where(on, np.sin(p) * 0.5 * (1 - np.cos(kp)), 0.0)
     u = (p / w - t0) / tw
+    if kind == abi.WAVE_GAUSS:  # EnvelopeTime of signal.hpp:55 (no carrier)
+        return np.exp(-u * u)
     return np.exp(-u * u) * np.sin(p)
 
 

@@ -43,7 +43,12 @@ enum { DTM_TFSF_NONE = 0, DTM_TFSF_DIPOLE = 1, DTM_TFSF_PLANE = 2 };
  * 0.5(1-cos(kEnv*phase))*sin(phase) on 0<=kEnv*phase<=2pi (signal.hpp:35,71-77);
  * GAUSS_SINE is exp(-((tau-t0)/tw)^2) sin(w tau) with tau=phase/w
  * (the envelope of signal.hpp:55-57).                                     */
-enum { DTM_WAVE_RAISED_COS = 0, DTM_WAVE_GAUSS_SINE = 1 };
+/* GAUSS is the reference's envelope pulse without a carrier,
+ * exp(-((tau-t0)/tw)^2) (EnvelopeTime/EnvelopeDiv1/EnvelopeDiv2,
+ * signal.hpp:55-57, used as the dipole moment in DipoleTFSF's alternative
+ * branch, signal.hpp:84-87): the broadband pulse of the reference's fdip.dat
+ * (its amplitude spectrum falls as exp(-w^2 tw^2/4) with tw = 0.1). */
+enum { DTM_WAVE_RAISED_COS = 0, DTM_WAVE_GAUSS_SINE = 1, DTM_WAVE_GAUSS = 2 };
 /* Absorbing layer: CPML (psi only in the slabs, kappa = 1, alpha = 0), or the
  * reference's split-field layer (genAlg.py:117-148): in the slabs each
  * component is the sum of one part per derivative axis, each part advanced

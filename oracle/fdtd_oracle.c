@@ -145,7 +145,9 @@ static real getf(const orc_sim* s, const real* F, int i, int j, int k) {
 
 /* Time function f(phase) and its first two phase-derivatives.
  * RAISED_COS: sin(p)*0.5*(1-cos(kEnv p)) on 0<=kEnv p<=2pi (signal.hpp:71-77).
- * GAUSS_SINE: exp(-((p/w-t0)/tw)^2) sin(p). */
+ * GAUSS_SINE: exp(-((p/w-t0)/tw)^2) sin(p).
+ * GAUSS: exp(-((p/w-t0)/tw)^2), the reference's EnvelopeTime pulse
+ * (signal.hpp:55-57; derivatives per phase, EnvelopeDiv1/2 divided by w, w^2). */
 static void wave3(int waveform, double p, double w, double kEnv, double t0, double tw,
                   double* f0, double* f1, double* f2) {
   if (waveform == DTM_WAVE_RAISED_COS) {
@@ -164,6 +166,12 @@ static void wave3(int waveform, double p, double w, double kEnv, double t0, doub
     double g = exp(-u * u);
     double g1 = -2 * u * a * g;
     double g2 = (4 * u * u * a * a - 2 * a * a) * g;
+    if (waveform == DTM_WAVE_GAUSS) { /* EnvelopeTime and its derivatives (signal.hpp:55-57), per phase */
+      *f0 = g;
+      *f1 = g1;
+      *f2 = g2;
+      return;
+    }
     double sp = sin(p), cp = cos(p);
     *f0 = g * sp;
     *f1 = g1 * sp + g * cp;

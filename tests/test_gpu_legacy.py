@@ -64,7 +64,7 @@ def legacy_config(cv):
     """The dtm_config that _main builds from the globals (dtm_legacy.cpp)."""
     sp = cv.shotpoint
     c = abi.default_config(cv.GridNx, cv.GridNy, cv.GridNz, cv.ds, cv.da, cv.dv, cv.dt)
-    c.npml[:] = [cv.Npmlx, cv.Npmlx, cv.Npmly, cv.Npmly, cv.Npmlz, cv.Npmlz]
+    c.npml[:] = [cv.Npmlx, cv.Npmlx, cv.Npmly, cv.Npmly, 0, cv.Npmlz]  # no layer at low z (DTmxw.cu:262)
     c.pml_vmax = sp.V_max if sp.V_max > 0 else 1.0
     c.ntime = cv.Ntime
     eps = [1.0, 5.0, 5.0, 4.0, 1e4, 1, 4, 9, 16, 25, 1, 1, 1.75 ** 2, 1.8 ** 2, 1.5 ** 2]

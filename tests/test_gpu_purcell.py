@@ -68,3 +68,24 @@ def test_dipole_box_too_small_for_the_singular_field_is_rejected():
                                       box_half=2)
     s = Solver(w.cfg)
     s.close()
+
+
+def test_dispersive_cells_in_the_cpml_are_rejected():
+    """A Drude cell inside a CPML slab makes the CPML-ADE update diverge (a
+    silver half-space in the layer grows without bound after ~2000 steps,
+    on the oracle as on the GPU), so dtm_set_materials refuses it; the same
+    medium with no layer on that face is accepted."""
+    from dtmaxwell4_amd.solver import Solver
+    w = configs.drude_lorentz_purcell(n=40, npml=6, radius=6, gap=4, nfreq=4, steps=10, cells_per_lambda=20)
+    slab = np.zeros((40, 40, 40), dtype=np.uint8)
+    slab[6:34, 6:34, :10] = 1                # the sphere's Drude+Lorentz medium below z = 10, inside
+                                             # the x and y layers, reaching into the low-z one
+    s = Solver(w.cfg)
+    with pytest.raises(abi.DTMError):
+        s.set_materials(slab, slab, slab)
+    s.close()
+    w.cfg.npml[4] = 0
+    s = Solver(w.cfg)
+    s.set_materials(slab, slab, slab)
+    s.step(5)
+    s.close()

@@ -38,7 +38,7 @@ def dipole_src(waveform):
     return t
 
 
-@pytest.mark.parametrize("waveform", [abi.WAVE_RAISED_COS, abi.WAVE_GAUSS_SINE])
+@pytest.mark.parametrize("waveform", [abi.WAVE_RAISED_COS, abi.WAVE_GAUSS_SINE, abi.WAVE_GAUSS])
 def test_dipole_field_satisfies_maxwell(waveform):
     src = dipole_src(waveform)
     c = src.cSrc

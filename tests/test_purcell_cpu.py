@@ -62,3 +62,27 @@ def test_dipole_spectrum_direct_sum():
         ref = sum(2.0 * orc.wave(t.waveform, t.w, 0.0, t.t0, t.tw, (n + 1) * dt) * np.exp(1j * wq * (n + 1) * dt) * dt
                   for n in range(steps))
         assert F[q] == pytest.approx(ref, rel=1e-12)
+
+
+def test_reference_dipole_moment_is_the_gaussian_envelope_pulse():
+    """The reference's fdip.dat is the spectrum of its EnvelopeTime pulse
+    d(t) = exp(-((t - t0)/tw)^2) (signal.hpp:55, no carrier) with the
+    TFSFsrc::set values t0 = 25*0.02, tw = 5*0.02 (signal.hpp:99-115): the
+    amplitude falls as exp(-w^2 tw^2 / 4) (measured 5e-6 from it over the
+    42 rows) and the phase as -w t0.  DTM_WAVE_GAUSS is that pulse: its
+    device spectrum has the same shape."""
+    data = json.load(open(os.path.join(HERE, "golden", "purcell_rows.json")))
+    w = np.array([r["omega"] for r in data["rows"]])
+    A = np.array([r["Ad"] for r in data["rows"]])
+    ph = np.unwrap(np.array([r["phd"] for r in data["rows"]]))
+    tw, t0 = 5 * 0.02, 25 * 0.02
+    shape = np.exp(-(w ** 2 - w[0] ** 2) * tw ** 2 / 4)
+    assert np.abs(A / A[0] / shape - 1).max() < 2e-5
+    assert np.polyfit(w, ph, 1)[0] == pytest.approx(-t0, abs=1e-4)
+    # a carrier (the Gaussian-modulated sine) would not fit
+    t = abi.Tfsf()
+    t.waveform, t.w, t.t0, t.tw, t.amp = abi.WAVE_GAUSS, 2 * np.pi / 0.5, t0, tw, 1.0
+    dt = 0.002
+    F = purcell.dipole_spectrum(t, w, 4000, dt)
+    assert np.abs(np.abs(F) / np.abs(F[0]) / shape - 1).max() < 1e-3
+    assert np.polyfit(w, np.unwrap(np.angle(F)), 1)[0] == pytest.approx(t0, abs=1e-3)  # device: e^{+iwt}
