@@ -180,20 +180,26 @@ def adjoint_run(steps, every=200):
                          f"checkpointed: state every {every} steps + recompute)")
 
 
-def purcell_run(steps):
+def purcell_run(steps, zone_margin=8):
     """Purcell error of FP16 storage on the dispersive dipole workload:
     384^3, 12-cell CPML, Drude (eps_inf 3.7, wp 1.37e16, gamma 2.7e13 rad/s) +
     Lorentz sphere r=40, Ez DipoleTFSF 4 cells above the surface, 64 FP64 DFT
-    bins on 0.5-1.5 f0.  Error = max_w |P16 - P32| / max_w |P32| (purcell.gp:2)."""
+    bins on 0.5-1.5 f0.  Error = max_w |P16 - P32| / max_w |P32| (purcell.gp:2).
+    FP16 runs twice: plain, and with an FP32 zone of the cells within
+    box_half + zone_margin of the dipole (dtm_set_fp32_zone)."""
     import numpy as np
 
     from dtmaxwell4_amd import abi, configs, purcell
     from dtmaxwell4_amd.solver import Solver
     out = {}
     curves = {}
-    for name, store in (("fp32", abi.STORE_FP32), ("fp16", abi.STORE_FP16)):
+    for name, store in (("fp32", abi.STORE_FP32), ("fp16", abi.STORE_FP16), ("fp16_zone", abi.STORE_FP16)):
         w = configs.drude_lorentz_purcell(steps=steps, storage=store)
-        s = configs.apply(w, Solver(w.cfg))
+        s = Solver(w.cfg)
+        if name == "fp16_zone":
+            s.set_fp32_zone(*configs.dipole_zone(w, zone_margin))
+            out["fp32_zone"] = list(configs.dipole_zone(w, zone_margin))
+        configs.apply(w, s)
         t = time.perf_counter()
         s.step(steps)
         s.sync()
@@ -206,8 +212,9 @@ def purcell_run(steps):
         out[name + "_gcells"] = cells * steps / el / 1e9
         log(f"[bench] purcell {name}: {out[name + '_gcells']:.2f} Gcells/s, P in "
             f"[{curves[name].min():.4g}, {curves[name].max():.4g}]")
-    P32, P16 = curves["fp32"], curves["fp16"]
+    P32, P16, P16z = curves["fp32"], curves["fp16"], curves["fp16_zone"]
     out["fp16_vs_fp32_error"] = float(np.max(np.abs(P16 - P32)) / np.max(np.abs(P32)))
+    out["fp16_zone_vs_fp32_error"] = float(np.max(np.abs(P16z - P32)) / np.max(np.abs(P32)))
     out["P_fp32_range"] = [float(P32.min()), float(P32.max())]
     # against the FP64 oracle's spectrum of the same workload (fixture written
     # by tests/golden/make_fullsize_fixtures.py: the oracle needs hours here)
@@ -217,7 +224,7 @@ def purcell_run(steps):
         w = configs.drude_lorentz_purcell(steps=steps)
         Fd = purcell.dipole_spectrum(w.cfg.tfsf, w.omegas, steps, w.cfg.dt)
         P64 = purcell.purcell_factor(w.omegas, d[0, :, 0] + 1j * d[0, :, 1], Fd)
-        for name, P in (("fp32", P32), ("fp16", P16)):
+        for name, P in (("fp32", P32), ("fp16", P16), ("fp16_zone", P16z)):
             out[f"{name}_vs_fp64_error"] = float(np.max(np.abs(P - P64)) / np.max(np.abs(P64)))
         out["fp64_reference"] = os.path.relpath(fx, ROOT)
     out["workload"] = (f"Drude+Lorentz sphere r=40 in 384^3, dipole TF/SF 4 cells from the surface, "
@@ -304,8 +311,9 @@ def main():
     pur = None
     if not args.no_purcell:
         pur = purcell_run(args.purcell_steps)
-        log(f"[bench] purcell fp16 vs fp32 error {pur['fp16_vs_fp32_error']:.3e}, vs fp64 "
-            f"{pur.get('fp16_vs_fp64_error')}")
+        log(f"[bench] purcell fp16 vs fp32 error {pur['fp16_vs_fp32_error']:.3e} (with the FP32 zone "
+            f"{pur['fp16_zone_vs_fp32_error']:.3e}), vs fp64 {pur.get('fp16_vs_fp64_error')} "
+            f"({pur.get('fp16_zone_vs_fp64_error')})")
     cpu = None
     if not args.no_cpu:
         try:
@@ -348,9 +356,13 @@ def main():
         "fp16_store": None if r16 is None else {
             "value": round(r16["gcells"], 3), "unit": "Gcells/s", "speedup_vs_fp32": round(r16["gcells"] / r32["gcells"], 3),
             "dominant_kernel_gbs": r16["groups"][r16["dominant"]]["gbs"],
-            "purcell_error": None if pur is None else pur.get("fp16_vs_fp64_error", pur["fp16_vs_fp32_error"]),
+            "purcell_error": None if pur is None else pur.get("fp16_zone_vs_fp64_error",
+                                                              pur["fp16_zone_vs_fp32_error"]),
             "purcell_error_basis": None if pur is None else (
-                "FP64 oracle" if "fp16_vs_fp64_error" in pur else "FP32 storage")},
+                ("FP64 oracle" if "fp16_zone_vs_fp64_error" in pur else "FP32 storage")
+                + ", FP16 storage with the FP32 zone around the dipole"),
+            "purcell_error_no_zone": None if pur is None else pur.get("fp16_vs_fp64_error",
+                                                                      pur["fp16_vs_fp32_error"])},
         "purcell": pur,
         "other_schedule_fp32": None if ralt is None else {
             "kernel": ralt["kernel"], "value": round(ralt["gcells"], 3), "unit": "Gcells/s",

@@ -166,6 +166,8 @@ def load_library():
         "dtm_energy": (C.c_int, [P, C.POINTER(C.c_double)]),
         "dtm_energy_box": (C.c_int, [P] + [C.c_int] * 6 + [C.POINTER(C.c_double)]),
         "dtm_add_flux_box": (C.c_int, [P] + [C.c_int] * 6),
+        "dtm_set_fp32_zone": (C.c_int, [P] + [C.c_int] * 6),
+        "dtm_set_tfsf_fold": (C.c_int, [P, C.c_int]),
         "dtm_flux_read": (C.c_int, [P, P, C.c_int]),
         "dtm_flux_total": (C.c_int, [P, P]),
         "dtm_write_arr": (C.c_int, [P, C.c_int, C.c_char_p]),

@@ -142,8 +142,11 @@ def plane_wave_sphere(nx=512, ny=256, nz=256, npml=12, inset=16, radius=40, eps=
     centre = (0.5 * nx * dx, 0.5 * ny * dx, 0.5 * nz * dx)
     mats = sphere_mats((nx, ny, nz), (dx, dx, dx), centre, radius * dx, 1)
     if storage == abi.STORE_FP16:
-        cfg.scale_e_log2[:] = [12, 12, 12]
-        cfg.scale_h_log2[:] = [12, 12, 12]
+        # the sphere focuses the pulse to |H| = 16 behind it (step ~1700 at
+        # the full size): 2^10 leaves the half range (65504 / 2^10 = 64) 4x
+        # above that
+        cfg.scale_e_log2[:] = [10, 10, 10]
+        cfg.scale_h_log2[:] = [10, 10, 10]
     c = (nx // 2, ny // 2, nz // 2)
     sensors = [("Ez", c[0] - radius - 4, c[1], c[2]), ("Ez", c[0] + radius + 4, c[1], c[2]),
                ("Ez", c[0], c[1], c[2])]
@@ -196,7 +199,20 @@ def drude_lorentz_purcell(n=384, npml=12, radius=40, gap=4, nfreq=64, steps=4000
     om = w0 * np.linspace(0.5, 1.5, nfreq)
     probes = [(abi.EZ, ci, ci, kd)]
     return Workload("drude_lorentz_purcell", cfg, mats=mats, probes=probes, omegas=om, steps=steps,
-                    notes=dict(dx=dx, dt=dt, w0=w0, dipole=(ci, ci, kd), pos=pos))
+                    notes=dict(dx=dx, dt=dt, w0=w0, dipole=(ci, ci, kd), pos=pos, box_half=box_half))
+
+
+def dipole_zone(w, margin=8):
+    """FP32 zone for FP16 storage around a dipole TF/SF box: the cells within
+    box_half + margin of the dipole cell (clipped to the grid), where the
+    total field is large; as (i0, i1, j0, j1, k0, k1)."""
+    c = w.notes["dipole"]
+    h = int(w.notes["box_half"]) + margin
+    n = (w.cfg.nx, w.cfg.ny, w.cfg.nz)
+    out = []
+    for a in range(3):
+        out += [max(c[a] - h, 0), min(c[a] + h + 1, n[a])]
+    return tuple(out)
 
 
 def gyro_slab(nx=256, ny=256, nz=512, npml=12, thick=64, steps=8000, lam0_um=1.5, cells_per_lambda=40,

@@ -125,8 +125,25 @@ template <> struct Fields<1> {
   __half* H[3];
 };
 
-struct Scales {  // FP16 storage: value = stored * inv, stored = value * scale
+// FP16 storage with an FP32 zone: a box of cells whose six components are
+// also kept in FP32.  Inside it every kernel reads and writes the FP32 copy;
+// the FP16 arrays receive its rounding, which is what cells outside the box
+// read.  Used around a dipole source, where the total field just outside
+// its TF/SF box is large and its FP16 rounding feeds the scattered field.
+struct Exact {
+  int on;
+  int i0, i1, j0, j1, k0, k1;
+  float* f[6];  // Ex .. Hz, index ((i - i0) * (j1 - j0) + (j - j0)) * (k1 - k0) + (k - k0)
+};
+constexpr long long kMaxZoneCells = 1LL << 27;
+__host__ __device__ __forceinline__ int exact_idx(const Exact& x, int i, int j, int k) {
+  if (!x.on || i < x.i0 || i >= x.i1 || j < x.j0 || j >= x.j1 || k < x.k0 || k >= x.k1) return -1;
+  return ((i - x.i0) * (x.j1 - x.j0) + (j - x.j0)) * (x.k1 - x.k0) + (k - x.k0);
+}
+
+struct Scales {  // FP16 storage: value = stored * inv, stored = value * scale, and the FP32 zone
   float se[3], sh[3], ie[3], ih[3];
+  Exact xz;
 };
 
 struct Region {  // a box of cells [i0,i1) x [j0,j1) x [k0,k1)
@@ -218,6 +235,7 @@ struct Soft {
   int n;
   int comp[16];
   long long q[16];
+  int ijk[16][3];  // cell of each source (the FP32 zone test)
   float val[16];  // dt * inv_eps * J(t)
 };
 
@@ -233,10 +251,17 @@ int take_launch_error(char* msg, int cap);
 
 // Launchers (dtm_kernels.hip).  All enqueue on `st`; each sweep is one
 // launch of the kernel variant with every boundary rule over `regs`.
+// fold non-null: add a plane-wave TF/SF source along x inside the sweep,
+// from this half-step's per-plane factor table (launch_tfsf_fold); ignored
+// with an FP32 zone
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Region* regs, int nreg, hipStream_t st);
+                     const Region* regs, int nreg, hipStream_t st, const float* fold = nullptr);
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st);
+                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st,
+                     const float* fold = nullptr);
+// floats per half-step of the fold factor table: (nx + 1) * 8
+void launch_tfsf_fold(const Tfsf& tf, const double* tables, float* out, int nsteps, const Grid& g,
+                      hipStream_t st);
 // TF/SF corrections applied after a sweep (half = 0: H, 1: E).  The update is
 // linear in the stencil sums, so the correction is additive, including its
 // CPML psi and ADE J terms.

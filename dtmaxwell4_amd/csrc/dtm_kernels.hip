@@ -39,6 +39,22 @@ __device__ __forceinline__ void st(float* F, long long q, float v, float) { F[q]
 __device__ __forceinline__ void st(__half* F, long long q, float v, float s) {
   F[q] = __float2half_rn(v * s);
 }
+// one cell of component comp (Ex .. Hz = 0 .. 5) with the FP32 zone: xq is
+// the cell's zone index (exact_idx) or -1; the half array always holds the
+// rounded zone value, so readers that ignore the zone see FP16 precision
+__device__ __forceinline__ float ldx(const float* F, long long q, float inv, const Exact&, int, int) {
+  return ld(F, q, inv);
+}
+__device__ __forceinline__ float ldx(const __half* F, long long q, float inv, const Exact& xz, int comp, int xq) {
+  return xq >= 0 ? xz.f[comp][xq] : ld(F, q, inv);
+}
+__device__ __forceinline__ void stx(float* F, long long q, float v, float s, const Exact&, int, int) {
+  st(F, q, v, s);
+}
+__device__ __forceinline__ void stx(__half* F, long long q, float v, float s, const Exact& xz, int comp, int xq) {
+  if (xq >= 0) xz.f[comp][xq] = v;
+  st(F, q, v, s);
+}
 template <typename T>
 __device__ __forceinline__ float ldn(const T* F, long long q, bool ok, float inv) {
   return ok ? ld(F, q, inv) : 0.0f;
@@ -107,18 +123,20 @@ __device__ __forceinline__ double incident(const Tfsf& s, int comp, int hx, int 
     n0 /= R;
     n1 /= R;
     n2 /= R;
-    // vectors along z: d = (0,0,d)
+    // dipole along the unit axis u (theta, phi; theta = 0: the reference's z
+    // dipole, signal.hpp:70-72): d = d u, so (dss x n) x n = dss ((u.n) n - u)
+    const double u0 = s.st * s.cph, u1 = s.st * s.sph, u2 = s.ct;
+    const double un = u0 * n0 + u1 * n1 + u2 * n2;
     if (comp < 3) {
-      // (dss x n) x n for dss = (0,0,dss): = dss*(n2*n - e_z)
       const double nc = comp == 0 ? n0 : (comp == 1 ? n1 : n2);
-      const double ez = comp == 2 ? 1.0 : 0.0;
-      const double t2 = dss * (n2 * nc - ez);
-      return c * (t2 / (c * c * R) + (3.0 * ds * n2 * nc - ds * ez) / (c * R * R) +
-                  (3.0 * d * n2 * nc - d * ez) / (R * R * R));
+      const double uc = comp == 0 ? u0 : (comp == 1 ? u1 : u2);
+      const double t2 = dss * (un * nc - uc);
+      return c * (t2 / (c * c * R) + (3.0 * ds * un * nc - ds * uc) / (c * R * R) +
+                  (3.0 * d * un * nc - d * uc) / (R * R * R));
     } else {
-      // (v e_z) x n = v (-n1, n0, 0)
+      // (v u) x n = v (u1 n2 - u2 n1, u2 n0 - u0 n2, u0 n1 - u1 n0)
       const int a = comp - 3;
-      const double cz = a == 0 ? -n1 : (a == 1 ? n0 : 0.0);
+      const double cz = a == 0 ? u1 * n2 - u2 * n1 : (a == 1 ? u2 * n0 - u0 * n2 : u0 * n1 - u1 * n0);
       return dss * cz / (c * c * R) + ds * cz / (c * R * R);
     }
   } else {
@@ -160,7 +178,8 @@ __device__ __forceinline__ int comp_at(int hx, int hy, int hz) {
 // unless the point lies inside the box on the other two axes; then a
 // neighbour crosses when its a-coordinate is on the other side of [hlo, hhi].
 template <bool TAB>
-__device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t) {
+__device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t,
+                                           const double* table) {
   const int hp[3] = {hx, hy, hz};
   const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
   if (hp[b1] < s.hlo[b1] || hp[b1] > s.hhi[b1] || hp[b2] < s.hlo[b2] || hp[b2] > s.hhi[b2]) return 0.f;
@@ -181,7 +200,7 @@ __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int h
     double v;
     if (TAB) {
       const int ha = s.tab_axis == a ? hq : hp[s.tab_axis];
-      v = s.P[cq] * s.table[min(max(ha - s.tab_h0, 0), s.tab_n - 1)];
+      v = s.P[cq] * table[min(max(ha - s.tab_h0, 0), s.tab_n - 1)];
     } else {
       v = incident(s, cq, a == 0 ? hq : hx, a == 1 ? hq : hy, a == 2 ? hq : hz, t);
     }
@@ -189,6 +208,68 @@ __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int h
   }
   return (float)acc;
 }
+
+// ------------------------------------------- TF/SF folded into the sweeps
+// A plane wave along x (table axis 0): the incident value at every stencil
+// neighbour along y or z of a component is P[cq] T(h_x) with h_x the
+// component's own x half-coordinate, so tfsf_dcorr along y or z separates into
+// a per-plane factor P[cq] T(h_x) [h_x inside the box] and a per-lane integer
+// weight W(h) = sum over the neighbours across the face of wt * (+-1).  Along
+// x it is a per-plane sum X(h_x) = sum wt * (+-1) T(h_x + o) times a per-lane
+// inside flag.  k_tfsf_fold writes the six per-plane factors of each half-step
+// (rd_a folded in); the sweeps add  weight * factor  to the six derivatives.
+__host__ __device__ __forceinline__ int fold_w(int h, int lo, int hi, int sf_inside) {
+  const bool pin = h >= lo && h <= hi;
+  int w = 0;
+  const int ho[4] = {-3, -1, 1, 3}, wt[4] = {1, -27, 27, -1};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool qin = h + ho[q] >= lo && h + ho[q] <= hi;
+    if (qin == pin) continue;
+    w += ((sf_inside ? qin : !qin) ? wt[q] : -wt[q]);
+  }
+  return w;  // |w| <= 28
+}
+// Six weights of one cell and half-step, packed: four signed 6-bit fields
+// (bits 0, 6, 12, 18: factors 0, 1, 2, 5) and two flags (bits 24, 25:
+// factors 3, 4).  H half-step (factors: Hx/y, Hx/z, Hy/z, Hy/x, Hz/x, Hz/y):
+//   W_y(2j) zin(2k), W_z(2k) yin(2j), W_z(2k) yin(2j+1), yin(2j+1) zin(2k),
+//   yin(2j) zin(2k+1), W_y(2j) zin(2k+1);
+// E half-step (Ex/y, Ex/z, Ey/z, Ey/x, Ez/x, Ez/y):
+//   W_y(2j+1) zin(2k+1), W_z(2k+1) yin(2j+1), W_z(2k+1) yin(2j), yin(2j) zin(2k+1),
+//   yin(2j+1) zin(2k), W_y(2j+1) zin(2k).
+// hd: hlo_y, hhi_y, hlo_z, hhi_z, sf_inside.
+__host__ __device__ __forceinline__ int fold_pack(int half, int j, int k, const int* hd) {
+  auto yin = [&](int h) { return h >= hd[0] && h <= hd[1] ? 1 : 0; };
+  auto zin = [&](int h) { return h >= hd[2] && h <= hd[3] ? 1 : 0; };
+  auto Wy = [&](int h) { return fold_w(h, hd[0], hd[1], hd[4]); };
+  auto Wz = [&](int h) { return fold_w(h, hd[2], hd[3], hd[4]); };
+  const int y0 = 2 * j, z0 = 2 * k;
+  int f0, f1, f2, f3, f4, f5;
+  if (half == 0) {
+    f0 = Wy(y0) * zin(z0);
+    f1 = Wz(z0) * yin(y0);
+    f2 = Wz(z0) * yin(y0 + 1);
+    f3 = yin(y0 + 1) * zin(z0);
+    f4 = yin(y0) * zin(z0 + 1);
+    f5 = Wy(y0) * zin(z0 + 1);
+  } else {
+    f0 = Wy(y0 + 1) * zin(z0 + 1);
+    f1 = Wz(z0 + 1) * yin(y0 + 1);
+    f2 = Wz(z0 + 1) * yin(y0);
+    f3 = yin(y0) * zin(z0 + 1);
+    f4 = yin(y0 + 1) * zin(z0);
+    f5 = Wy(y0 + 1) * zin(z0);
+  }
+  return (f0 & 63) | ((f1 & 63) << 6) | ((f2 & 63) << 12) | ((f5 & 63) << 18) | (f3 << 24) | (f4 << 25);
+}
+__device__ __forceinline__ float fold_f(int w, int off) {  // signed 6-bit field at bit off
+  return (float)((int)((unsigned)w << (26 - off)) >> 26);
+}
+__device__ __forceinline__ float fold_b(int w, int bit) { return (float)((w >> bit) & 1); }
+// Floats per plane in the factor table; the table of one half-step starts
+// with kFoldStride ints of header (hlo_y, hhi_y, hlo_z, hhi_z, sf_inside).
+constexpr int kFoldStride = 8;
 
 // -------------------------------------------------------- CPML helpers
 __device__ __forceinline__ int slab_index(int v, int n, int plo, int phi) {
@@ -344,7 +425,7 @@ template <int STORE> struct Raw;
 template <> struct Raw<0> { using T = float; };
 template <> struct Raw<1> { using T = __half; };
 __device__ __forceinline__ float cvt(float v, float) { return v; }
-__device__ __forceinline__ float cvt(__half v, float inv) { return __half2float(v) * inv; }
+__device__ __forceinline__ float cvt(__half v, float inv) { return __builtin_fmaf(__half2float(v), inv, 0.0f); }
 template <typename T> __device__ __forceinline__ T rzero();
 template <> __device__ __forceinline__ float rzero<float>() { return 0.0f; }
 template <> __device__ __forceinline__ __half rzero<__half>() { return __ushort_as_half((unsigned short)0); }
@@ -376,11 +457,17 @@ template <> struct Vec<0, 1> { using P = float; };
 template <> struct Vec<1, 1> { using P = __half; };
 template <> struct Vec<1, 2> { using P = __half2; };
 __device__ __forceinline__ void unpackv(float p, float, float* o) { o[0] = p; }
-__device__ __forceinline__ void unpackv(__half p, float inv, float* o) { o[0] = __half2float(p) * inv; }
+__device__ __forceinline__ void unpackv(__half p, float inv, float* o) {
+  o[0] = __builtin_fmaf(__half2float(p), inv, 0.0f);
+}
+// The conversions are written as fma(x, scale, 0): one mixed-precision
+// v_fma_mix (half operand, float result) per value instead of a convert and
+// a multiply.  The scale is a power of two, so the product is exact and the
+// result is the same as the convert-then-multiply.
 __device__ __forceinline__ void unpackv(__half2 p, float inv, float* o) {
   const float2 f = __half22float2(p);
-  o[0] = f.x * inv;
-  o[1] = f.y * inv;
+  o[0] = __builtin_fmaf(f.x, inv, 0.0f);
+  o[1] = __builtin_fmaf(f.y, inv, 0.0f);
 }
 template <typename P> __device__ __forceinline__ P pzero();
 template <> __device__ __forceinline__ float pzero<float>() { return 0.0f; }
@@ -389,10 +476,11 @@ template <> __device__ __forceinline__ __half2 pzero<__half2>() { return __half2
 // store V values at element q (q even for V = 2)
 __device__ __forceinline__ void stv(float* F, long long q, const float* v, float) { F[q] = v[0]; }
 __device__ __forceinline__ void stv(__half* F, long long q, const float* v, float s) {
-  F[q] = __float2half_rn(v[0] * s);
+  F[q] = __float2half_rn(__builtin_fmaf(v[0], s, 0.0f));
 }
 __device__ __forceinline__ void stv2(__half* F, long long q, const float* v, float s) {
-  *reinterpret_cast<__half2*>(F + q) = __floats2half2_rn(v[0] * s, v[1] * s);
+  *reinterpret_cast<__half2*>(F + q) =
+      __floats2half2_rn(__builtin_fmaf(v[0], s, 0.0f), __builtin_fmaf(v[1], s, 0.0f));
 }
 // word of V cells at a position under the boundary rule (see at())
 template <typename P, typename T>
@@ -414,8 +502,13 @@ template <typename P, typename T> struct PreH {
 // bit 1: y CPML (per row).  The y and z boundary rules and the z CPML are
 // always on: they are per-thread constants.
 // SP: split-field absorbing layer instead of the CPML (one cell per lane).
-template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8>
-__global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms) {
+// TF: a plane-wave TF/SF source along x folded in: the correction of
+// genAlg.py:219-221 is added to the derivative it belongs to before the CPML
+// and split-field updates, which are linear in it (the k_tfsf kernel applies
+// the same terms after a sweep); fold: this half-step's factor table.
+template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false>
+__global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms,
+                                                         const float* fold) {
   using T = typename Raw<STORE>::T;
   using P = typename Vec<STORE, V>::P;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
@@ -448,6 +541,26 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
 
   const long long hq = inreg ? own : -1;  // H operands: owned cells only
+  // FP32 zone (FP16 storage): a block whose tile (with its halo) meets the
+  // zone replaces the LDS values of zone cells by their FP32 copies and
+  // writes its new zone values there too
+  static_assert(!XZ || STORE == 1, "the FP32 zone applies to FP16 storage");
+  const Exact& xz = sc.xz;
+  bool xtile = false;
+  if constexpr (XZ)
+    xtile = xz.on && (g.per[1] || g.per[2] ||
+                      (xz.j0 < t.j0 + MBY + 3 && xz.j1 > t.j0 - 3 && xz.k0 < t.k0 + BZV + 3 && xz.k1 > t.k0 - 3));
+  const int jo = j;
+  const int jy = yw ? nbr(t.j0 + (ty < 2 ? ty - 2 : MBY), 0, g.ny, g.per[1]) : -1;
+  const int kz = zl ? nbr(t.k0 + (tx < 2 ? tx - 2 : BZV), 0, g.nz, g.per[2]) : -1;
+  auto ov = [&](float& dst, int comp, int pi, int pj, int pk) {
+    // rows and lanes past the grid's edge hold the wrapped cells on a
+    // periodic axis (plane_off), nothing otherwise
+    pj = nbr(pj, 0, g.ny, g.per[1]);
+    pk = nbr(pk, 0, g.nz, g.per[2]);
+    const int xq = (pi < 0 || pj < 0 || pk < 0) ? -1 : exact_idx(xz, pi, pj, pk);
+    if (xq >= 0) dst = xz.f[comp][xq];
+  };
   auto fetch = [&](PreH<P, T>& S, int p) {
     const long long pb = plane_base<GX>(g, p);
     const long long pn = plane_base<GX>(g, p + 1);
@@ -479,6 +592,18 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
       for (int q = 0; q < V; q++) rEz[s][yrow][c0 + q] = v[q];
     }
     if (zl) rEy[s][ty][zcol] = cvt(ldr(Ey, pb, zh), iey);
+    if constexpr (XZ) {
+      if (xtile) {
+        const int pp = GX ? nbr(p, 0, g.nx, g.per[0]) : p;
+#pragma unroll
+        for (int q = 0; q < V; q++) {
+          ov(rEy[s][ty][c0 + q + 2], 1, pp, jo, k + q);
+          ov(rEz[s][ty + 2][c0 + q], 2, pp, jo, k + q);
+          if (yw) ov(rEz[s][yrow][c0 + q], 2, pp, jy, k + q);
+        }
+        if (zl) ov(rEy[s][ty][zcol], 1, pp, jo, kz);
+      }
+    }
   };
 
   const int ysl = (GY && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
@@ -493,6 +618,15 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
     cz[q] = zs[q] >= 0 ? pm.cH[2][k + q] : 0.f;
   }
   const float by = ysl >= 0 ? pm.bH[1][j] : 0.f, cy = ysl >= 0 ? pm.cH[1][j] : 0.f;
+  // TF/SF fold: this lane's packed weights (zero away from the y and z faces)
+  int tfw[V];
+#pragma unroll
+  for (int q = 0; q < V; q++) tfw[q] = 0;
+  if constexpr (TF) {
+    const int* hd = reinterpret_cast<const int*>(fold);
+#pragma unroll
+    for (int q = 0; q < V; q++) tfw[q] = fold_pack(0, j, k + q, hd);
+  }
 
   // PF prefetch sets, each a plane ahead of the next: three planes in flight
   // where the registers allow it (the y-CPML variant keeps two).
@@ -549,6 +683,29 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
     unpackv(Q.hz, ihz, v);
 #pragma unroll
     for (int q = 0; q < V; q++) sHo[2][ty][c0 + q] = v[q];
+    if constexpr (XZ) {
+      if (xtile && (g.per[0] || (i + 1 >= xz.i0 && i - 1 < xz.i1))) {  // Ex of plane i, Ey / Ez of i+1, own H of i
+        const int ii = GX ? nbr(i, 0, g.nx, g.per[0]) : i;
+        const int i1n = GX ? nbr(i + 1, 0, g.nx, g.per[0]) : i + 1;
+#pragma unroll
+        for (int q = 0; q < V; q++) {
+          ov(sEx[b][ty + 2][c0 + q + 2], 0, ii, jo, k + q);
+          ov(rEy[s3][ty][c0 + q + 2], 1, i1n, jo, k + q);
+          ov(rEz[s3][ty + 2][c0 + q], 2, i1n, jo, k + q);
+          ov(sHo[0][ty][c0 + q], 3, ii, jo, k + q);
+          ov(sHo[1][ty][c0 + q], 4, ii, jo, k + q);
+          ov(sHo[2][ty][c0 + q], 5, ii, jo, k + q);
+          if (yw) {
+            ov(sEx[b][yrow][c0 + q + 2], 0, ii, jy, k + q);
+            ov(rEz[s3][yrow][c0 + q], 2, i1n, jy, k + q);
+          }
+        }
+        if (zl) {
+          ov(sEx[b][ty + 2][zcol], 0, ii, jo, kz);
+          ov(rEy[s3][ty][zcol], 1, i1n, jo, kz);
+        }
+      }
+    }
     // CPML psi of plane i, loaded before the prefetch so that waiting for
     // psi does not wait for it (loads complete in issue order)
     // (z and y slabs; the x slab is a few planes, its psi is read in place)
@@ -567,6 +724,15 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
       }
     }
     const int xsl = (GX && pm.on) ? slab_index(i, g.nx, pm.plo[0], pm.phi[0]) : -1;
+    // this plane's fold factors, loaded before the barrier (whose wait covers
+    // them) rather than inside the band branch
+    float fu[6];
+#pragma unroll
+    for (int t6 = 0; t6 < 6; t6++) fu[t6] = 0.f;
+    if constexpr (TF) {
+#pragma unroll
+      for (int t6 = 0; t6 < 6; t6++) fu[t6] = fold[kFoldStride * (1 + i) + t6];
+    }
     fetch(Q, min(i + PF, t.i1 - 1));
     lds_barrier();
     if (inreg) {
@@ -582,6 +748,17 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
         float dzEx = rd2 * d24(sEx[b][ty + 2][c], sEx[b][ty + 2][c + 1], exo, sEx[b][ty + 2][c + 3]);
         float dxEz = rd0 * d24(rEz[s0][ty + 2][c], rEz[s1][ty + 2][c], rEz[s2][ty + 2][c], rEz[s3][ty + 2][c]);
         float dxEy = rd0 * d24(rEy[s0][ty][c + 2], rEy[s1][ty][c + 2], rEy[s2][ty][c + 2], rEy[s3][ty][c + 2]);
+        if constexpr (TF) {
+          if (tfw[q]) {  // a lane next to a y or z face (or inside the box on both)
+            const int w = tfw[q];
+            dyEz += fold_f(w, 0) * fu[0];   // Hx, y faces
+            dzEy += fold_f(w, 6) * fu[1];   // Hx, z faces
+            dzEx += fold_f(w, 12) * fu[2];  // Hy, z faces
+            dxEz += fold_b(w, 24) * fu[3];  // Hy, x faces
+            dxEy += fold_b(w, 25) * fu[4];  // Hz, x faces
+            dyEx += fold_f(w, 18) * fu[5];  // Hz, y faces
+          }
+        }
         if constexpr (SP) {
           // split parts, (k1, k2) = (1, 1) off the slab: Hx (y, z), Hy (z, x), Hz (x, y)
           const bool sx = GX && xsl >= 0;
@@ -612,6 +789,19 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
         }
       }
       const long long qq = plane_base<GX>(g, i) + own;
+      if constexpr (XZ) {
+        if (xtile) {  // new H of zone cells to the FP32 copy
+#pragma unroll
+          for (int q = 0; q < V; q++) {
+            const int xq = exact_idx(xz, i, j, k + q);
+            if (xq >= 0) {
+              xz.f[3][xq] = nx_[q];
+              xz.f[4][xq] = ny_[q];
+              xz.f[5][xq] = nz_[q];
+            }
+          }
+        }
+      }
       if constexpr (V == 2) {
         stv2(Hx, qq, nx_, sc.sh[0]);
         stv2(Hy, qq, ny_, sc.sh[1]);
@@ -652,8 +842,9 @@ __global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Fields<STO
       if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) continue;
     }
     float curl[3] = {ade.curl[0][b], ade.curl[1][b], ade.curl[2][b]};
+    const int xq = STORE == 1 ? exact_idx(sc.xz, i, j, k) : -1;
     float En[3];
-    for (int c = 0; c < 3; c++) En[c] = ld(f.E[c], q, sc.ie[c]);
+    for (int c = 0; c < 3; c++) En[c] = ldx(f.E[c], q, sc.ie[c], sc.xz, c, xq);
     if ((ce.special & 8) && ade.gyro_on) {
       const GyroMat& gm = ade.gmats[ce.mat[0]];
       const float J0[3] = {ade.Jg[0][b], ade.Jg[1][b], ade.Jg[2][b]};
@@ -681,9 +872,9 @@ __global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Fields<STO
           ade.Jp[c][p][b] = jn;
         }
         ade.Em[c][b] = Ep;
-        st(fd.E[c], q, Enew, sc.se[c]);
+        stx(fd.E[c], q, Enew, sc.se[c], sc.xz, c, xq);
       } else {
-        st(fd.E[c], q, En[c] + g.dt * ce.inv_eps[c] * curl[c], sc.se[c]);
+        stx(fd.E[c], q, En[c] + g.dt * ce.inv_eps[c] * curl[c], sc.se[c], sc.xz, c, xq);
       }
     }
   }
@@ -751,7 +942,7 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
       const int slot = t == 0 ? 1 : 0;
       const int comp = HALF == 0 ? 3 + c : c;
       const int hx = 2 * i + off[comp][0], hy = 2 * j + off[comp][1], hz = 2 * k + off[comp][2];
-      const float D = g.rd[a] * tfsf_dcorr<TAB>(tf, hx, hy, hz, a, tinc);
+      const float D = g.rd[a] * tfsf_dcorr<TAB>(tf, hx, hy, hz, a, tinc, tf.table);
       if (D == 0.0f) continue;
       float cp = 0.0f;
       if (pm.on && pm.split) {  // split-field layer: the part of axis a gets k2 * D
@@ -786,9 +977,11 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
       dc[c] += sgn * (1.0f + cp) * D;
     }
   }
+  const int xq = STORE == 1 ? exact_idx(sc.xz, i, j, k) : -1;
   if (HALF == 0) {
     for (int c = 0; c < 3; c++) {
-      if (dc[c] != 0.0f) st(f.H[c], q, ld(f.H[c], q, sc.ih[c]) - g.dt * dc[c], sc.sh[c]);
+      if (dc[c] != 0.0f)
+        stx(f.H[c], q, ldx(f.H[c], q, sc.ih[c], sc.xz, 3 + c, xq) - g.dt * dc[c], sc.sh[c], sc.xz, 3 + c, xq);
       if (dc2[c] != 0.0f) pm.p2H[c][q] -= g.dt * dc2[c];
     }
     return;
@@ -809,11 +1002,11 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc
       const long long b = (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k - ade.k0);
       const AdeMat& am = ade.mats[ce.mat[c]];
       const float dE = am.kE[2] * g.dt * dc[c];
-      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + dE, sc.se[c]);
+      stx(f.E[c], q, ldx(f.E[c], q, sc.ie[c], sc.xz, c, xq) + dE, sc.se[c], sc.xz, c, xq);
       for (int p = 0; p < am.np; p++) ade.Jp[c][p][b] += am.kJ[p][2] * dE;
     } else {
       const float ie = over ? dv : ce.inv_eps[c];
-      st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) + g.dt * ie * dc[c], sc.se[c]);
+      stx(f.E[c], q, ldx(f.E[c], q, sc.ie[c], sc.xz, c, xq) + g.dt * ie * dc[c], sc.se[c], sc.xz, c, xq);
       if (dc2[c] != 0.0f) pm.p2E[c][q] += g.dt * ie * dc2[c];
     }
   }
@@ -853,9 +1046,9 @@ template <int V> struct ClsWord;
 template <> struct ClsWord<1> { using T = uint8_t; };
 template <> struct ClsWord<2> { using T = unsigned short; };
 
-template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8>
+template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false>
 __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
-                                                         Ade ade, MarchSet ms) {
+                                                         Ade ade, MarchSet ms, const float* fold) {
   using T = typename Raw<STORE>::T;
   using P = typename Vec<STORE, V>::P;
   using CL = typename ClsWord<V>::T;
@@ -893,6 +1086,24 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
 
   const long long eq = inreg ? own : -1;  // E operands: owned cells only
+  // FP32 zone (FP16 storage), as in the H sweep
+  static_assert(!XZ || STORE == 1, "the FP32 zone applies to FP16 storage");
+  const Exact& xz = sc.xz;
+  bool xtile = false;
+  if constexpr (XZ)
+    xtile = xz.on && (g.per[1] || g.per[2] ||
+                      (xz.j0 < t.j0 + MBY + 3 && xz.j1 > t.j0 - 3 && xz.k0 < t.k0 + BZV + 3 && xz.k1 > t.k0 - 3));
+  const int jo = j;
+  const int jy = yw ? nbr(t.j0 + (ty == 0 ? -1 : MBY + ty - 1), 0, g.ny, g.per[1]) : -1;
+  const int kz = zl ? nbr(t.k0 + (tx == 0 ? -1 : BZV + tx - 1), 0, g.nz, g.per[2]) : -1;
+  auto ov = [&](float& dst, int comp, int pi, int pj, int pk) {
+    // rows and lanes past the grid's edge hold the wrapped cells on a
+    // periodic axis (plane_off), nothing otherwise
+    pj = nbr(pj, 0, g.ny, g.per[1]);
+    pk = nbr(pk, 0, g.nz, g.per[2]);
+    const int xq = (pi < 0 || pj < 0 || pk < 0) ? -1 : exact_idx(xz, pi, pj, pk);
+    if (xq >= 0) dst = xz.f[comp][xq];
+  };
   // with no class array every cell is class 0: read the zero page
   const uint8_t* clsp = mt.cls ? mt.cls : static_cast<const uint8_t*>(g.zero);
   const long long cq = mt.cls ? eq : 0;
@@ -928,6 +1139,18 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
       for (int q = 0; q < V; q++) rHz[s][yrow][c0 + q] = v[q];
     }
     if (zl) rHy[s][ty][zcol] = cvt(ldr(Hy, pb, zh), ihy);
+    if constexpr (XZ) {
+      if (xtile) {
+        const int pp = GX ? nbr(p, 0, g.nx, g.per[0]) : p;
+#pragma unroll
+        for (int q = 0; q < V; q++) {
+          ov(rHy[s][ty][c0 + q + 1], 4, pp, jo, k + q);
+          ov(rHz[s][ty + 1][c0 + q], 5, pp, jo, k + q);
+          if (yw) ov(rHz[s][yrow][c0 + q], 5, pp, jy, k + q);
+        }
+        if (zl) ov(rHy[s][ty][zcol], 4, pp, jo, kz);
+      }
+    }
   };
 
   const int ysl = (GY && pm.on) ? slab_index(j, g.ny, pm.plo[1], pm.phi[1]) : -1;
@@ -944,6 +1167,15 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
   }
   const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
   const float by = ysl >= 0 ? pm.bE[1][j] : 0.f, cy = ysl >= 0 ? pm.cE[1][j] : 0.f;
+  // TF/SF fold: this lane's packed weights (zero away from the y and z faces)
+  int tfw[V];
+#pragma unroll
+  for (int q = 0; q < V; q++) tfw[q] = 0;
+  if constexpr (TF) {
+    const int* hd = reinterpret_cast<const int*>(fold);
+#pragma unroll
+    for (int q = 0; q < V; q++) tfw[q] = fold_pack(1, j, k + q, hd);
+  }
 
   constexpr int PF = (GY || V == 2) ? 2 : 3;  // prefetch depth, as in the H sweep
   PreE<P, T, CL> S[PF];
@@ -996,6 +1228,29 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
     for (int q = 0; q < V; q++) sEo[2][ty][c0 + q] = v[q];
 #pragma unroll
     for (int q = 0; q < V; q++) sCl[ty][c0 + q] = (uint8_t)(Q.cls >> (8 * q));
+    if constexpr (XZ) {
+      if (xtile && (g.per[0] || (i + 2 >= xz.i0 && i < xz.i1))) {  // Hx and own E of plane i, Hy / Hz of i+2
+        const int ii = GX ? nbr(i, 0, g.nx, g.per[0]) : i;
+        const int i2n = GX ? nbr(i + 2, 0, g.nx, g.per[0]) : i + 2;
+#pragma unroll
+        for (int q = 0; q < V; q++) {
+          ov(sHx[b][ty + 1][c0 + q + 1], 3, ii, jo, k + q);
+          ov(rHy[s3][ty][c0 + q + 1], 4, i2n, jo, k + q);
+          ov(rHz[s3][ty + 1][c0 + q], 5, i2n, jo, k + q);
+          ov(sEo[0][ty][c0 + q], 0, ii, jo, k + q);
+          ov(sEo[1][ty][c0 + q], 1, ii, jo, k + q);
+          ov(sEo[2][ty][c0 + q], 2, ii, jo, k + q);
+          if (yw) {
+            ov(sHx[b][yrow][c0 + q + 1], 3, ii, jy, k + q);
+            ov(rHz[s3][yrow][c0 + q], 5, i2n, jy, k + q);
+          }
+        }
+        if (zl) {
+          ov(sHx[b][ty + 1][zcol], 3, ii, jo, kz);
+          ov(rHy[s3][ty][zcol], 4, i2n, jo, kz);
+        }
+      }
+    }
     // psi and design-box operands of plane i before the prefetch
     float pz0[V], pz1[V], py0[V], py1[V], dv[V];
     bool over[V];
@@ -1016,6 +1271,15 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
         dv[q] = mt.d_inv_eps[((long long)(i - mt.d_i0) * mt.d_nj + (j - mt.d_j0)) * mt.d_nk + (k + q - mt.d_k0)];
     }
     const int xsl = (GX && pm.on) ? slab_index(i, g.nx, pm.plo[0], pm.phi[0]) : -1;
+    // this plane's fold factors, loaded before the barrier (whose wait covers
+    // them) rather than inside the band branch
+    float fu[6];
+#pragma unroll
+    for (int t6 = 0; t6 < 6; t6++) fu[t6] = 0.f;
+    if constexpr (TF) {
+#pragma unroll
+      for (int t6 = 0; t6 < 6; t6++) fu[t6] = fold[kFoldStride * (1 + i) + t6];
+    }
     fetch(Q, min(i + PF, t.i1 - 1));
     lds_barrier();
     if (inreg) {
@@ -1033,6 +1297,17 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
         float dzHx = rd2 * d24(sHx[b][ty + 1][c], hxo, sHx[b][ty + 1][c + 2], sHx[b][ty + 1][c + 3]);
         float dxHz = rd0 * d24(rHz[s0][ty + 1][c], rHz[s1][ty + 1][c], rHz[s2][ty + 1][c], rHz[s3][ty + 1][c]);
         float dxHy = rd0 * d24(rHy[s0][ty][c + 1], rHy[s1][ty][c + 1], rHy[s2][ty][c + 1], rHy[s3][ty][c + 1]);
+        if constexpr (TF) {
+          if (tfw[q]) {
+            const int w = tfw[q];
+            dyHz += fold_f(w, 0) * fu[0];   // Ex, y faces
+            dzHy += fold_f(w, 6) * fu[1];   // Ex, z faces
+            dzHx += fold_f(w, 12) * fu[2];  // Ey, z faces
+            dxHz += fold_b(w, 24) * fu[3];  // Ey, x faces
+            dxHy += fold_b(w, 25) * fu[4];  // Ez, x faces
+            dyHx += fold_f(w, 18) * fu[5];  // Ez, y faces
+          }
+        }
         if (!SP && zs[q] >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
           cpml_step(pm.psiE[2][0], zb + zs[q], pz0[q], bz[q], cz[q], dzHy);
           cpml_step(pm.psiE[2][1], zb + zs[q], pz1[q], bz[q], cz[q], dzHx);
@@ -1077,6 +1352,19 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
         }
       }
       const long long qq = plane_base<GX>(g, i) + own;
+      if constexpr (XZ) {
+        if (xtile) {  // new E of zone cells to the FP32 copy (a dispersive cell keeps its old value for k_ade)
+#pragma unroll
+          for (int q = 0; q < V; q++) {
+            const int xq = exact_idx(xz, i, j, k + q);
+            if (xq >= 0) {
+              xz.f[0][xq] = nE[0][q];
+              xz.f[1][xq] = nE[1][q];
+              xz.f[2][xq] = nE[2][q];
+            }
+          }
+        }
+      }
       if constexpr (V == 2) {
         // a pair with a dispersive cell stores only its plain cells: k_ade
         // writes the other one (from the old value it reads)
@@ -1119,7 +1407,8 @@ __global__ void k_soft(Fields<STORE> f, Scales sc, Soft so) {
   if (t >= so.n) return;
   const int c = so.comp[t];
   const long long q = so.q[t];
-  st(f.E[c], q, ld(f.E[c], q, sc.ie[c]) - so.val[t], sc.se[c]);
+  const int xq = STORE == 1 ? exact_idx(sc.xz, so.ijk[t][0], so.ijk[t][1], so.ijk[t][2]) : -1;
+  stx(f.E[c], q, ldx(f.E[c], q, sc.ie[c], sc.xz, c, xq) - so.val[t], sc.se[c], sc.xz, c, xq);
 }
 
 template <int STORE>
@@ -1127,7 +1416,9 @@ __device__ float fval(const Grid& g, const Fields<STORE>& f, const Scales& sc, i
   const int ii = nbr(i, 0, g.nx, g.per[0]), jj = nbr(j, 0, g.ny, g.per[1]), kk = nbr(k, 0, g.nz, g.per[2]);
   if (ii < 0 || jj < 0 || kk < 0) return 0.f;
   const long long q = ii * g.sx + jj * g.sy + kk;
-  return comp < 3 ? ld(f.E[comp], q, sc.ie[comp]) : ld(f.H[comp - 3], q, sc.ih[comp - 3]);
+  const int xq = STORE == 1 ? exact_idx(sc.xz, ii, jj, kk) : -1;
+  return comp < 3 ? ldx(f.E[comp], q, sc.ie[comp], sc.xz, comp, xq)
+                  : ldx(f.H[comp - 3], q, sc.ih[comp - 3], sc.xz, comp, xq);
 }
 
 // Reference sensor interpolation (sensor.h:64-71) on the cell lattice.
@@ -1267,56 +1558,57 @@ static bool use_16_rows(int slots8, int slots16, const Region* regs, int nreg, i
   return c16 <= c8;
 }
 
-template <int STORE, int V, bool SP>
+template <int STORE, int V, bool SP, bool XZ, bool TF>
 static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
-                  hipStream_t st) {
+                  const float* fold, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   int seg8 = 0, seg16 = 0;
   if constexpr (V == 1) {
-    static const int slots8 = resident_blocks(k_update_h<STORE, 3, V, SP, 8>, BZ * 8);
-    static const int slots16 = resident_blocks(k_update_h<STORE, 3, V, SP, 16>, BZ * 16);
+    static const int slots8 = resident_blocks(k_update_h<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
+    static const int slots16 = resident_blocks(k_update_h<STORE, 3, V, SP, 16, XZ, TF>, BZ * 16);
     if (use_16_rows(slots8, slots16, regs, nreg, BZ, seg8, seg16)) {
       MarchSet rg = make_march(regs, nreg, seg16, BZ, 16);
       if (rg.total <= 0) return;
-      hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 16>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
-                         rg);
+      hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 16, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
+                         rg, fold);
       check_launch("k_update_h");
       return;
     }
   } else {
-    static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP, 8>, BZ * 8);
+    static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
     seg8 = march_seg(regs, nreg, slots, BZ * V, 8);
   }
   MarchSet rg = make_march(regs, nreg, seg8, BZ * V, 8);
   if (rg.total <= 0) return;
-  hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 8>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, rg);
+  hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 8, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, rg,
+                     fold);
   check_launch("k_update_h");
 }
 
-template <int STORE, int V, bool SP>
+template <int STORE, int V, bool SP, bool XZ, bool TF>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
-                  const Ade& ade, const Region* regs, int nreg, hipStream_t st) {
+                  const Ade& ade, const Region* regs, int nreg, const float* fold, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   int seg8 = 0, seg16 = 0;
   if constexpr (V == 1) {
-    static const int slots8 = resident_blocks(k_update_e<STORE, 3, V, SP, 8>, BZ * 8);
-    static const int slots16 = resident_blocks(k_update_e<STORE, 3, V, SP, 16>, BZ * 16);
+    static const int slots8 = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
+    static const int slots16 = resident_blocks(k_update_e<STORE, 3, V, SP, 16, XZ, TF>, BZ * 16);
     if (use_16_rows(slots8, slots16, regs, nreg, BZ, seg8, seg16)) {
       MarchSet rg = make_march(regs, nreg, seg16, BZ, 16);
       if (rg.total <= 0) return;
-      hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 16>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
-                         mt, ade, rg);
+      hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 16, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
+                         mt, ade, rg, fold);
       check_launch("k_update_e");
       return;
     }
   } else {
-    static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP, 8>, BZ * 8);
+    static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
     seg8 = march_seg(regs, nreg, slots, BZ * V, 8);
   }
   MarchSet rg = make_march(regs, nreg, seg8, BZ * V, 8);
   if (rg.total <= 0) return;
-  hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 8>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, mt,
-                     ade, rg);
+  hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 8, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, mt,
+                     ade, rg, fold);
   check_launch("k_update_e");
 }
 
@@ -1326,32 +1618,66 @@ static bool pair_cells(int store, const Grid& g, const Materials* mt) {
   return store == 1 && g.nz % 2 == 0 && !(mt && mt->d_on);
 }
 
-void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Region* regs, int nreg, hipStream_t st) {
+// The sweep variant of a configuration: storage, cells per lane, layer
+// kind, FP32 zone (its own one-cell-per-lane variants, so that the other
+// variants carry none of its registers) and a folded TF/SF source (tf
+// non-null: a tabulated plane wave; never with the zone).
+template <bool TF>
+static void update_h_go(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                        const Region* regs, int nreg, const float* fold, hipStream_t st) {
+  const bool xz = store == 1 && sc.xz.on;
   if (pm.split) {  // split-field layer: one cell per lane
-    if (store == 0) upd_h<0, 1, true>(g, fields, sc, pm, regs, nreg, st);
-    else upd_h<1, 1, true>(g, fields, sc, pm, regs, nreg, st);
+    if (store == 0) upd_h<0, 1, true, false, TF>(g, fields, sc, pm, regs, nreg, fold, st);
+    else if (xz) upd_h<1, 1, true, true, false>(g, fields, sc, pm, regs, nreg, fold, st);
+    else upd_h<1, 1, true, false, TF>(g, fields, sc, pm, regs, nreg, fold, st);
   } else if (store == 0) {
-    upd_h<0, 1, false>(g, fields, sc, pm, regs, nreg, st);
+    upd_h<0, 1, false, false, TF>(g, fields, sc, pm, regs, nreg, fold, st);
+  } else if (xz) {
+    if (pair_cells(store, g, nullptr)) upd_h<1, 2, false, true, false>(g, fields, sc, pm, regs, nreg, fold, st);
+    else upd_h<1, 1, false, true, false>(g, fields, sc, pm, regs, nreg, fold, st);
   } else if (pair_cells(store, g, nullptr)) {
-    upd_h<1, 2, false>(g, fields, sc, pm, regs, nreg, st);
+    upd_h<1, 2, false, false, TF>(g, fields, sc, pm, regs, nreg, fold, st);
   } else {
-    upd_h<1, 1, false>(g, fields, sc, pm, regs, nreg, st);
+    upd_h<1, 1, false, false, TF>(g, fields, sc, pm, regs, nreg, fold, st);
+  }
+}
+
+void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                     const Region* regs, int nreg, hipStream_t st, const float* fold) {
+  if (fold && !(store == 1 && sc.xz.on))
+    update_h_go<true>(store, g, fields, sc, pm, regs, nreg, fold, st);
+  else
+    update_h_go<false>(store, g, fields, sc, pm, regs, nreg, nullptr, st);
+}
+
+template <bool TF>
+static void update_e_go(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
+                        const Materials& mt, const Ade& ade, const Region* regs, int nreg, const float* fold,
+                        hipStream_t st) {
+  const bool xz = store == 1 && sc.xz.on;
+  if (pm.split) {
+    if (store == 0) upd_e<0, 1, true, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else if (xz) upd_e<1, 1, true, true, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else upd_e<1, 1, true, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+  } else if (store == 0) {
+    upd_e<0, 1, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+  } else if (xz) {
+    if (pair_cells(store, g, &mt)) upd_e<1, 2, false, true, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else upd_e<1, 1, false, true, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+  } else if (pair_cells(store, g, &mt)) {
+    upd_e<1, 2, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+  } else {
+    upd_e<1, 1, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
   }
 }
 
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st) {
-  if (pm.split) {
-    if (store == 0) upd_e<0, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, st);
-    else upd_e<1, 1, true>(g, fields, sc, pm, mt, ade, regs, nreg, st);
-  } else if (store == 0) {
-    upd_e<0, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, st);
-  } else if (pair_cells(store, g, &mt)) {
-    upd_e<1, 2, false>(g, fields, sc, pm, mt, ade, regs, nreg, st);
-  } else {
-    upd_e<1, 1, false>(g, fields, sc, pm, mt, ade, regs, nreg, st);
-  }
+                     const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st,
+                     const float* fold) {
+  if (fold && !(store == 1 && sc.xz.on))
+    update_e_go<true>(store, g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+  else
+    update_e_go<false>(store, g, fields, sc, pm, mt, ade, regs, nreg, nullptr, st);
 }
 
 // One launch over the union of the face bands (k_tfsf gives each cell one
@@ -1392,6 +1718,82 @@ void launch_tfsf_tables(const Tfsf& tf, double* out, long long n0, int nsteps, d
   const long long n = (long long)nsteps * 2 * tf.tab_n;
   hipLaunchKernelGGL(k_tfsf_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tf, out, n0, nsteps, dt);
   check_launch("k_tfsf_table");
+}
+
+// Per-plane factors of the folded TF/SF source (plane wave along x) for the
+// half-steps of a table batch: out[(s*2 + half)*(nx+1)*kFoldStride] holds the
+// header, then plane i's six factors at +kFoldStride*(1+i).  tables: the
+// batch's waveform tables (k_tfsf_table).  With T(h) = P-less table value at
+// x half-coordinate h, xin(h) = h inside the box on x and
+// X(h) = sum over the neighbours h+o across an x face of wt_o (+-1) T(h+o):
+//   H half: rd1 xin(2i+1) P[Ez] T(2i+1), rd2 xin(2i+1) P[Ey] T(2i+1),
+//           rd2 xin(2i) P[Ex] T(2i),     rd0 P[Ez] X(2i), rd0 P[Ey] X(2i),
+//           rd1 xin(2i) P[Ex] T(2i);
+//   E half: rd1 xin(2i) P[Hz] T(2i),     rd2 xin(2i) P[Hy] T(2i),
+//           rd2 xin(2i+1) P[Hx] T(2i+1), rd0 P[Hz] X(2i+1), rd0 P[Hy] X(2i+1),
+//           rd1 xin(2i+1) P[Hx] T(2i+1).
+__global__ __launch_bounds__(256) void k_tfsf_fold(Tfsf tf, const double* tables, float* out, int nsteps, int nx,
+                                                   float rd0, float rd1, float rd2) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = nx + 1;
+  if (idx >= nsteps * 2 * per) return;
+  const int sh = idx / per, p = idx % per;  // sh = step * 2 + half
+  const int half = sh & 1;
+  float* o = out + (long long)sh * per * kFoldStride;
+  if (p == 0) {  // header
+    int* hd = reinterpret_cast<int*>(o);
+    hd[0] = tf.hlo[1];
+    hd[1] = tf.hhi[1];
+    hd[2] = tf.hlo[2];
+    hd[3] = tf.hhi[2];
+    hd[4] = tf.sf_inside;
+    hd[5] = hd[6] = hd[7] = 0;
+    return;
+  }
+  const int i = p - 1;
+  const double* T = tables + (long long)sh * tf.tab_n;
+  auto Tv = [&](int h) { return T[min(max(h - tf.tab_h0, 0), tf.tab_n - 1)]; };
+  auto xin = [&](int h) { return h >= tf.hlo[0] && h <= tf.hhi[0] ? 1.0 : 0.0; };
+  auto X = [&](int h) {
+    const bool pin = h >= tf.hlo[0] && h <= tf.hhi[0];
+    const int ho[4] = {-3, -1, 1, 3};
+    const double wt[4] = {1.0, -27.0, 27.0, -1.0};
+    double acc = 0.0;
+    for (int q = 0; q < 4; q++) {
+      const int hq = h + ho[q];
+      const bool qin = hq >= tf.hlo[0] && hq <= tf.hhi[0];
+      if (qin == pin) continue;
+      const double v = Tv(hq);
+      acc += wt[q] * ((tf.sf_inside ? qin : !qin) ? v : -v);
+    }
+    return acc;
+  };
+  const int e = 2 * i, od = 2 * i + 1;
+  float* u = o + kFoldStride * p;
+  if (half == 0) {  // neighbours are E components (0, 1, 2)
+    u[0] = (float)(rd1 * xin(od) * tf.P[2] * Tv(od));
+    u[1] = (float)(rd2 * xin(od) * tf.P[1] * Tv(od));
+    u[2] = (float)(rd2 * xin(e) * tf.P[0] * Tv(e));
+    u[3] = (float)(rd0 * tf.P[2] * X(e));
+    u[4] = (float)(rd0 * tf.P[1] * X(e));
+    u[5] = (float)(rd1 * xin(e) * tf.P[0] * Tv(e));
+  } else {          // neighbours are H components (3, 4, 5)
+    u[0] = (float)(rd1 * xin(e) * tf.P[5] * Tv(e));
+    u[1] = (float)(rd2 * xin(e) * tf.P[4] * Tv(e));
+    u[2] = (float)(rd2 * xin(od) * tf.P[3] * Tv(od));
+    u[3] = (float)(rd0 * tf.P[5] * X(od));
+    u[4] = (float)(rd0 * tf.P[4] * X(od));
+    u[5] = (float)(rd1 * xin(od) * tf.P[3] * Tv(od));
+  }
+  u[6] = u[7] = 0.f;
+}
+
+void launch_tfsf_fold(const Tfsf& tf, const double* tables, float* out, int nsteps, const Grid& g,
+                      hipStream_t st) {
+  const int n = nsteps * 2 * (g.nx + 1);
+  hipLaunchKernelGGL(k_tfsf_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tf, tables, out, nsteps,
+                     g.nx, g.rd[0], g.rd[1], g.rd[2]);
+  check_launch("k_tfsf_fold");
 }
 
 void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,

@@ -63,6 +63,8 @@ struct dtm_solver {
   Cpml pm{};         // current psi
   Cpml pmn{};        // second psi buffers (torre step)
   bool torre = false; // fused single-pass step; false: two sweeps per step
+  bool no_fold = false; // tabulated plane-wave TF/SF through k_tfsf instead of the sweeps
+  float* fold_base = nullptr; // per-plane factors of a folded plane wave, kTableSteps x 2 half-steps
   // device copies of the fused step's pointer table, one per buffer parity
   // (the old / new buffers swap every step), and their host images
   TorreCold* cold_dev[2] = {};
@@ -121,6 +123,9 @@ struct dtm_solver {
   double grp_ms[5] = {};
   long long grp_cnt[5] = {};
 };
+
+// floats of one half-step's fold factor table: header + nx planes, 8 each
+static long long fold_floats(const dtm_solver* s) { return ((long long)s->cfg.nx + 1) * 8; }
 
 template <typename T>
 static int dalloc(dtm_solver* s, T** p, size_t bytes) {
@@ -463,8 +468,12 @@ static int tfsf_plan(const dtm_config& c, Tfsf& t, TfsfBands& B) {
 
 static int setup_tfsf(dtm_solver* s) {
   if (int rc = tfsf_plan(s->cfg, s->tf, s->bands)) return rc;
-  if (s->tf.kind != DTM_TFSF_NONE && s->tf.tab_axis >= 0)
+  if (s->tf.kind != DTM_TFSF_NONE && s->tf.tab_axis >= 0) {
     if (int rc = dalloc(s, &s->tab_base, sizeof(double) * 2 * kTableSteps * s->tf.tab_n)) return rc;
+    // per-plane factors of a wave along x, for the sweeps that fold it in
+    if (s->tf.tab_axis == 0 && !s->torre)
+      if (int rc = dalloc(s, &s->fold_base, sizeof(float) * 2 * kTableSteps * fold_floats(s))) return rc;
+  }
   return 0;
 }
 
@@ -477,9 +486,15 @@ static Tfsf tfsf_for(dtm_solver* s, long long n, int half) {
   if (s->tab_n0 < 0 || n < s->tab_n0 || n >= s->tab_n0 + kTableSteps) {
     s->tab_n0 = n;
     launch_tfsf_tables(s->tf, s->tab_base, n, kTableSteps, s->cfg.dt, s->stream);
+    if (s->fold_base) launch_tfsf_fold(s->tf, s->tab_base, s->fold_base, kTableSteps, s->g, s->stream);
   }
   t.table = s->tab_base + ((n - s->tab_n0) * 2 + half) * (long long)t.tab_n;
   return t;
+}
+
+// The fold factor table of step n, half (after tfsf_for filled its batch).
+static const float* fold_for(const dtm_solver* s, long long n, int half) {
+  return s->fold_base + ((n - s->tab_n0) * 2 + half) * (long long)fold_floats(s);
 }
 
 static bool tfsf_gate(const dtm_solver* s, long long n) {
@@ -1100,23 +1115,32 @@ static int one_step(dtm_solver* s) {
     if (s->tf.active)
       launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
   } else {
+    // A plane wave along x is folded into the sweeps (k_tfsf's terms added to
+    // the derivatives); other plane waves, the dipole and the FP32 zone keep
+    // k_tfsf.
+    const bool fold = s->tf.active && s->fold_base && !s->sc.xz.on && !s->no_fold;
     // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
+    Tfsf tfh{};
+    if (s->tf.active) tfh = tfsf_for(s, n, 0);
     ev_begin(s, 0);
-    launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, s->stream);
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, s->stream, fold ? fold_for(s, n, 0) : nullptr);
     ev_end(s);
-    if (s->tf.active) {
+    if (s->tf.active && !fold) {
       ev_begin(s, 4);
-      launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 0), s->bands, th, s->stream);
+      launch_tfsf(s->store, 0, s->g, F, s->sc, s->pm, s->mt, s->ade, tfh, s->bands, th, s->stream);
       ev_end(s);
     }
     // E half-step: incident H at t = (n+1/2) dt (genAlg.py:499)
+    Tfsf tfe{};
+    if (s->tf.active) tfe = tfsf_for(s, n, 1);
     ev_begin(s, 2);
-    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, &s->whole, 1, s->stream);
+    launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, &s->whole, 1, s->stream,
+                    fold ? fold_for(s, n, 1) : nullptr);
     ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, F, s->sc, s->mt, s->ade, s->stream);
-    if (s->tf.active)
-      launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
+    if (s->tf.active && !fold)
+      launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfe, s->bands, te, s->stream);
   }
   if (c.nsoft > 0 && !s->adjoint) {
     Soft so{};
@@ -1135,6 +1159,9 @@ static int one_step(dtm_solver* s) {
       const double J = ss.amp * wave_value(ss.waveform, ss.w, ss.kEnv, ss.t0, ss.tw, th);
       so.comp[q] = ss.comp;
       so.q[q] = (long long)idx;
+      so.ijk[q][0] = ss.i;
+      so.ijk[q][1] = ss.j;
+      so.ijk[q][2] = ss.k;
       so.val[q] = (float)(c.dt * ie * J);
     }
     launch_soft(s->store, s->g, F, s->sc, so, s->stream);
@@ -1467,6 +1494,31 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
   return rc;
 }
 
+// The FP32 zone's values of component comp into (to_host) or from a host
+// array of the whole grid.
+static int zone_copy(dtm_solver* s, int comp, float* buf, bool to_host) {
+  const Exact& x = s->sc.xz;
+  if (!x.on) return 0;
+  const dtm_config& c = s->cfg;
+  const int ni = x.i1 - x.i0, nj = x.j1 - x.j0, nk = x.k1 - x.k0;
+  std::vector<float> z((size_t)ni * nj * nk);
+  if (!to_host)
+    for (int i = 0; i < ni; i++)
+      for (int j = 0; j < nj; j++)
+        memcpy(&z[((size_t)i * nj + j) * nk], &buf[((size_t)(x.i0 + i) * c.ny + (x.j0 + j)) * c.nz + x.k0],
+               sizeof(float) * nk);
+  const hipMemcpyKind kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+  HIPCHK(hipMemcpyAsync(to_host ? (void*)z.data() : (void*)x.f[comp], to_host ? (const void*)x.f[comp] : z.data(),
+                        sizeof(float) * z.size(), kind, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (to_host)
+    for (int i = 0; i < ni; i++)
+      for (int j = 0; j < nj; j++)
+        memcpy(&buf[((size_t)(x.i0 + i) * c.ny + (x.j0 + j)) * c.nz + x.k0], &z[((size_t)i * nj + j) * nk],
+               sizeof(float) * nk);
+  return 0;
+}
+
 int dtm_field_read(dtm_solver* s, int comp, float* buf) {
   if (!s || !buf || comp < 0 || comp > 5) return fail(DTM_ERR_ARG, "bad field read arguments");
   const size_t n = (size_t)s->cfg.nx * s->cfg.ny * s->cfg.nz;
@@ -1481,7 +1533,7 @@ int dtm_field_read(dtm_solver* s, int comp, float* buf) {
     HIPCHK(hipMemcpyAsync(buf, s->tmp_dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
-  return 0;
+  return zone_copy(s, comp, buf, true);
 }
 
 // ------------------------------------------------------ save / resume
@@ -1516,6 +1568,7 @@ static std::vector<std::pair<void*, size_t>> canonical_state(const dtm_solver* s
     }
     add(s->ade.Jg[c]);
   }
+  for (int c = 0; c < 6; c++) add(s->sc.xz.f[c]);  // the FP32 zone
   return out;
 }
 
@@ -1647,6 +1700,7 @@ int dtm_field_write(dtm_solver* s, int comp, const float* buf) {
     const float sc = comp < 3 ? s->sc.se[comp] : s->sc.sh[comp - 3];
     launch_from_float(1, s->tmp_dev, sc, comp_ptr(s, comp), (long long)n, s->stream);
     if (int rc = launch_result()) return rc;
+    if (int rc = zone_copy(s, comp, const_cast<float*>(buf), false)) return rc;
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
@@ -1727,6 +1781,35 @@ int dtm_add_flux_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int 
     }
   }
   return id;
+}
+
+int dtm_set_tfsf_fold(dtm_solver* s, int on) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  s->no_fold = !on;
+  return 0;
+}
+
+int dtm_set_fp32_zone(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1) {
+  if (!s) return fail(DTM_ERR_ARG, "null solver");
+  if (s->started) return fail(DTM_ERR_STATE, "the FP32 zone must be set before the first step");
+  if (s->store != DTM_STORE_FP16) return fail(DTM_ERR_ARG, "the FP32 zone applies to FP16 storage only");
+  if (s->torre) return fail(DTM_ERR_ARG, "the FP32 zone runs with the two-pass sweeps only");
+  if (s->sc.xz.on) return fail(DTM_ERR_STATE, "the FP32 zone is already set");
+  if (!box_ok(s, i0, i1, j0, j1, k0, k1)) return fail(DTM_ERR_ARG, "FP32 zone outside the grid or empty");
+  const long long cells = (long long)(i1 - i0) * (j1 - j0) * (k1 - k0);
+  if (cells > kMaxZoneCells) return fail(DTM_ERR_LIMIT, "FP32 zone larger than 2^27 cells");
+  Exact x{};
+  for (int c = 0; c < 6; c++)
+    if (int rc = sdalloc(s, &x.f[c], sizeof(float) * (size_t)cells)) return rc;
+  x.i0 = i0;
+  x.i1 = i1;
+  x.j0 = j0;
+  x.j1 = j1;
+  x.k0 = k0;
+  x.k1 = k1;
+  x.on = 1;
+  s->sc.xz = x;
+  return 0;
 }
 
 int dtm_flux_read(dtm_solver* s, double* out, int nsteps_max) {

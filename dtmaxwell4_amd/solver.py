@@ -158,6 +158,16 @@ class Solver:
             abi.check(self.L.dtm_energy_box(self.h, *[int(b) for b in box], C.byref(v)), "dtm_energy_box")
         return v.value
 
+    def set_tfsf_fold(self, on=True):
+        """Tabulated plane-wave TF/SF inside the sweeps (default) or in the
+        separate correction kernel; same fields."""
+        abi.check(self.L.dtm_set_tfsf_fold(self.h, int(bool(on))), "dtm_set_tfsf_fold")
+
+    def set_fp32_zone(self, i0, i1, j0, j1, k0, k1):
+        """FP16 storage: keep the cells [i0,i1) x [j0,j1) x [k0,k1) in FP32 too
+        (around a dipole source and its probes); before the first step."""
+        abi.check(self.L.dtm_set_fp32_zone(self.h, i0, i1, j0, j1, k0, k1), "dtm_set_fp32_zone")
+
     def add_flux_box(self, i0, i1, j0, j1, k0, k1):
         """Record the FP64 Poynting flux through the six faces of a box every
         step; returns the box id."""

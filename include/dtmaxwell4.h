@@ -138,7 +138,9 @@ typedef struct dtm_tfsf {
   double kEnv;     /* raised-cosine envelope ratio Omega/w */
   double cSrc;     /* phase velocity used in the source */
   double Imp;      /* impedance E/H of the incident wave */
-  double theta, phi, psi; /* plane wave: propagation angles, polarisation */
+  double theta, phi, psi; /* plane wave: propagation angles, polarisation;
+                            dipole: axis (sin th cos ph, sin th sin ph, cos th),
+                            th = 0 the reference's z dipole (signal.hpp:70-72) */
   double t0, tw;   /* Gaussian envelope centre and width */
   double amp;      /* amplitude (the reference's A=1) */
   double tStop;    /* corrections are applied while the gate allows it */
@@ -274,6 +276,28 @@ int dtm_energy_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1
  * counterpart in the reference; for callers that cannot allocate (code
  * under stream capture, tight serving loops). */
 int dtm_energy_box_noalloc(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1, double* out);
+
+/* Where the two-pass schedule applies a plane-wave TF/SF source whose k lies
+ * along an axis (incident field from a per-step table): on (the default)
+ * inside the H and E sweeps, added to the derivatives the face terms of
+ * genAlg.py:219-221 belong to; off: in a separate correction kernel after
+ * each sweep.  Both give the same fields (the updates are linear in those
+ * terms).  A dipole or an oblique plane wave, the fused step and the FP32
+ * zone always use the separate kernel. */
+int dtm_set_tfsf_fold(dtm_solver* s, int on);
+
+/* ------------------------------------------------------------ FP32 zone */
+/* FP16 storage with a box of cells [i0,i1) x [j0,j1) x [k0,k1) whose six
+ * components are also kept in FP32: every kernel that reads or writes a
+ * field value of a zone cell uses the FP32 copy (the half array holds its
+ * rounded value, so readers that do not know the zone, such as the flux and
+ * energy sums, see FP16 precision there).  Put it around a dipole TF/SF box
+ * and its probes: the total field there is large and its FP16 rounding
+ * otherwise dominates a Purcell factor's error.  FP16 storage and the
+ * two-pass sweeps only, set before the first step, once, at most 2^27 cells.
+ * dtm_field_read / dtm_field_write and the state files include the zone.
+ * No counterpart in the reference (its fields are FP32 ftype, params.h:26-29). */
+int dtm_set_fp32_zone(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1);
 
 /* ------------------------------------------------------- Poynting flux */
 /* FP64 Poynting-flux sums through the six faces of a box of cells

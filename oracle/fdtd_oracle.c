@@ -198,9 +198,15 @@ double orc_incident(const dtm_tfsf* src, int comp, double x, double y, double z,
     double phase = w * t - w / c * R;
     double f0, f1, f2;
     wave3(src->waveform, phase, w, src->kEnv, src->t0, src->tw, &f0, &f1, &f2);
-    double d[3] = {0, 0, src->amp * f0};
-    double ds[3] = {0, 0, src->amp * w * f1};
-    double dss[3] = {0, 0, src->amp * w * w * f2};
+    /* dipole axis u = (sin th cos ph, sin th sin ph, cos th); th = 0 is the
+     * reference's z dipole (signal.hpp:70-72) */
+    double u[3] = {sin(src->theta) * cos(src->phi), sin(src->theta) * sin(src->phi), cos(src->theta)};
+    double d[3], ds[3], dss[3];
+    for (int q = 0; q < 3; q++) {
+      d[q] = src->amp * f0 * u[q];
+      ds[q] = src->amp * w * f1 * u[q];
+      dss[q] = src->amp * w * w * f2 * u[q];
+    }
     n[0] /= R; n[1] /= R; n[2] /= R;
     double t1[3], t2[3];
     if (comp < 3) {

@@ -47,7 +47,9 @@ def gpu_run(name, storage=abi.STORE_FP32, nx=None, ny=None):
     if storage == abi.STORE_FP16:
         w.cfg.storage = abi.STORE_FP16
         s = w.cfg.scale_e_log2
-        if name == "drude_lorentz_purcell":
+        # half range 65504 / 2^scale: 64 for the plane wave (its focus behind
+        # the sphere reaches |H| = 16) and the dipole, 16 for the slab
+        if name in ("drude_lorentz_purcell", "plane_wave_sphere"):
             w.cfg.scale_e_log2[:], w.cfg.scale_h_log2[:] = [10] * 3, [10] * 3
         else:
             w.cfg.scale_e_log2[:], w.cfg.scale_h_log2[:] = [12] * 3, [12] * 3

@@ -38,9 +38,14 @@ def dipole_src(waveform):
     return t
 
 
-@pytest.mark.parametrize("waveform", [abi.WAVE_RAISED_COS, abi.WAVE_GAUSS_SINE, abi.WAVE_GAUSS])
-def test_dipole_field_satisfies_maxwell(waveform):
+@pytest.mark.parametrize("waveform,axis", [(abi.WAVE_RAISED_COS, (0.0, 0.0)), (abi.WAVE_GAUSS_SINE, (0.0, 0.0)),
+                                           (abi.WAVE_GAUSS, (0.0, 0.0)), (abi.WAVE_GAUSS, (np.pi / 2, 0.0)),
+                                           (abi.WAVE_GAUSS_SINE, (1.1, 2.3))])
+def test_dipole_field_satisfies_maxwell(waveform, axis):
+    """The dipole's E and B (any axis: theta = pi/2, phi = 0 is an x dipole)
+    obey Faraday's and Ampere's laws in vacuum."""
     src = dipole_src(waveform)
+    src.theta, src.phi = axis
     c = src.cSrc
     f = lambda comp, r, t: orc.incident(src, comp, r[0], r[1], r[2], t)
     h = 1e-4
