@@ -325,7 +325,7 @@ def main():
     achieved = dom["gbs"]
     # measured HBM bytes of the dominant kernel, from the committed profile
     prefix = {"E sweep": "void dtm::k_update_e<0, 3, 1,", "H sweep": "void dtm::k_update_h<0, 3, 1,",
-              "fused step": "void dtm::k_torre<0, 3>"}.get(r32["dominant"], "")
+              "fused step": "void dtm::k_torre2<0, 3>"}.get(r32["dominant"], "")
     tr = profiled_traffic(prefix, sw) if prefix and (nx, ny, nz) == (512, 256, 256) else None
     out = {
         "metric": "Gcells/s at 512x256x256 FP32 vs FP16-store; adjoint/forward time ratio",

@@ -1116,8 +1116,8 @@ static int one_step(dtm_solver* s) {
       launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
   } else {
     // A plane wave along x is folded into the sweeps (k_tfsf's terms added to
-    // the derivatives); other plane waves, the dipole and the FP32 zone keep
-    // k_tfsf.
+    // the derivatives, from per-plane factors); other plane waves, the dipole
+    // and a plane wave with the FP32 zone keep k_tfsf.
     const bool fold = s->tf.active && s->fold_base && !s->sc.xz.on && !s->no_fold;
     // H half-step: incident E at t = n dt (genAlg.py:220 with time 0)
     Tfsf tfh{};
@@ -1856,7 +1856,7 @@ int dtm_time_steps(dtm_solver* s, int nsteps, double* ms) {
 }
 
 const char* dtm_kernel_name(const dtm_solver* s) {
-  if (s && s->torre) return s->store == 0 ? "k_torre<fp32>" : "k_torre<fp16>";
+  if (s && s->torre) return s->store == 0 ? "k_torre2<fp32>" : "k_torre2<fp16>";
   return kernel_variant_name(s ? s->store : 0);
 }
 

@@ -12,19 +12,11 @@
 // Block = 64 lanes (z) x TR=16 rows (y).  On every plane p the block
 // computes H on a 16 x 67 tile, rows j0-1 .. j0+14: lane tx at column
 // k0+tx, and the first wave (row 0, which owns no E) also at the halo
-// columns k0-1, k0+64, k0+65 of all 16 rows.  Two planes behind
-// (m = p-2) it computes E on the owned 13 x 64 tile (rows j0 .. j0+12, every
-// lane).  Halo H values are recomputed by neighbouring blocks from the same
-// old fields, so fields and CPML psi are double-buffered (read old, write
-// new) and blocks need no synchronisation with each other.
-//
-// Pipeline: the operands of plane p+1 are loaded while plane p computes
-// (PF = 1: the block holds 1024 lanes and 157 KB of LDS, one block per CU,
-// so 128 VGPRs is the budget and a second operand set would spill), with
-// unconditional loads (positions outside the grid read the zero pad); the
-// x neighbours come from LDS rings of E tiles (Ex: 4 planes, Ey/Ez: 5) and
-// of new H tiles (5 planes); the H read-modify-write operands and E class
-// bytes pass through per-thread LDS slots; one LDS-only barrier per plane.
+// columns k0-1, k0+64, k0+65 of all 16 rows.  Two planes behind it computes
+// E on the owned 13 x 64 tile (rows j0 .. j0+12, every lane).  Halo H values
+// are recomputed by neighbouring blocks from the same old fields, so fields
+// and CPML psi are double-buffered (read old, write new) and blocks need no
+// synchronisation with each other.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -49,7 +41,6 @@ constexpr int ER = TR + 3;        // E tile rows (19): j0-3 .. j0+15
 constexpr int EC = TZ + 6;        // E tile cols (70): k0-3 .. k0+66
 constexpr int ETILE = ER * EC;    // 1330
 constexpr int NXCD = 8;
-constexpr int PF = 1;             // prefetch depth (planes); 2 spills at 128 VGPRs
 
 template <typename T> __device__ __forceinline__ float cv(T v, float inv);
 template <> __device__ __forceinline__ float cv<float>(float v, float) { return v; }
@@ -128,360 +119,11 @@ __device__ __forceinline__ void cpml_d(float po, float* pn, unsigned pq, float b
   D += v;
 }
 
-// Operands of one plane p, fetched PF planes ahead: the Ex tile values of
-// plane p and the Ey / Ez tile values of plane p+1 (two flat positions per
-// thread), the old H of plane p at this thread's main and halo columns, and
-// the class byte of plane p-2 (the E plane of that iteration).
-template <typename T> struct PreT {
-  T ex[2], ey[2], ez[2];
-  T hx, hy, hz, gx, gy, gz;
-  uint8_t cls;
-};
 
 }  // namespace
 
-// GEN bit 0: x boundary rule and x CPML (per plane); bit 1: y CPML.
-template <int STORE, int GEN>
-__global__ __launch_bounds__(TZ* TR) void k_torre(Grid g, Scales sc, const uint8_t* clsg, const TorreCold* cold,
-                                                  TorreSet ts) {
-  using T = typename std::conditional<STORE == 0, float, __half>::type;
-  constexpr bool GX = GEN & 1, GY = GEN & 2;
-  __shared__ ClassEntry tbl[kMaxClasses];
-  __shared__ float rEx[4][ETILE];         // E tile rings: Ex planes p-3..p
-  __shared__ float rEy[5][ETILE];         // Ey, Ez planes p-3..p+1
-  __shared__ float rEz[5][ETILE];
-  __shared__ float rH[5][3][TR][HC];      // new H tiles, planes p-4..p
-  __shared__ float sHo[3][TR][HC];        // this thread's old H of plane p
-  __shared__ uint8_t sCl[TR][TZ];         // this thread's E class byte of plane p-2
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int tid = ty * TZ + tx;
-  const ColdK* C0 = (const ColdK*)cold;  // set-up reads
-  static_assert(sizeof(ClassEntry) == 16, "class entries are copied as four words");
-  for (int q = tid; q < 4 * C0->mt.nclass; q += TZ * TR)
-    reinterpret_cast<unsigned*>(tbl)[q] = gl(reinterpret_cast<const unsigned*>(C0->mt.table))[q];
-  TorreTile t;
-  if (!torre_locate(ts, t)) return;  // block-uniform
-  const Region& R = ts.r[t.r];
-
-  // ---- H positions of this thread: main column tx+1 of row ty; the first
-  // wave (row 0, which owns no E) also takes the 3 halo columns of every row,
-  // lane l at row l/3, so the other waves run one H evaluation per plane
-  const int jh = t.j0 - 1 + ty;
-  const int jm = nbr(jh, g.ny, g.per[1]);
-  const int khm = t.k0 + tx;  // main H column
-  const bool hal = ty == 0 && tx < 3 * TR;
-  const int hr = hal ? tx / 3 : 0;  // halo row
-  const int chh = tx % 3 == 0 ? 0 : (tx % 3 == 1 ? HC - 2 : HC - 1);  // halo tile column
-  const int khh = t.k0 - 1 + chh;
-  const int jmh = nbr(t.j0 - 1 + hr, g.ny, g.per[1]);
-  const int kmm = nbr(khm, g.nz, g.per[2]), kmh = nbr(khh, g.nz, g.per[2]);
-  const bool hv_m = jm >= 0 && kmm >= 0, hv_h = hal && jmh >= 0 && kmh >= 0;
-  const int ho_m = hv_m ? jm * (int)g.sy + kmm : -1;  // in-plane offsets (ny*nz < 2^31)
-  const int ho_h = hv_h ? jmh * (int)g.sy + kmh : -1;
-  // stored: rows j0..j0+12 inside the region, main column inside the region
-  const bool rown = ty >= 1 && ty < TR - 2 && jh >= R.j0 && jh < R.j1;
-  const bool hown_c = rown && khm >= R.k0 && khm < R.k1;
-  const bool eown = hown_c;  // E at (jh, khm) for the same rows and column
-
-  // ---- flat E-tile positions loaded by this thread (up to two)
-  int foff[2];
-  int fpos[2];
-  for (int u = 0; u < 2; u++) {
-    const int p = tid + u * TZ * TR;
-    fpos[u] = p < ETILE ? p : -1;
-    foff[u] = -1;
-    if (p < ETILE) {
-      const int r = p / EC, c = p % EC;
-      const int jj = nbr(t.j0 - 3 + r, g.ny, g.per[1]), kk = nbr(t.k0 - 3 + c, g.nz, g.per[2]);
-      if (jj >= 0 && kk >= 0) foff[u] = jj * (int)g.sy + kk;
-    }
-  }
-  const int ho_e = (eown && clsg) ? ho_m : -1;  // class byte of the owned E cell
-  const uint8_t* clsp = clsg ? clsg : static_cast<const uint8_t*>(g.zero);
-  const unsigned czidx = clsg ? g.zidx : 0u;
-
-  const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
-  const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
-
-  auto plane = [&](int i) -> long long {  // plane base under the x boundary rule (-1: zero)
-    if (!GX) return (long long)i * g.sx;
-    const int ii = nbr(i, g.nx, g.per[0]);
-    return ii < 0 ? -1LL : (long long)ii * g.sx;
-  };
-  auto fetch = [&](PreT<T>& S, int p, const ColdK* C) {
-    const T* Exo = static_cast<const T*>(C->fo[0]);
-    const T* Eyo = static_cast<const T*>(C->fo[1]);
-    const T* Ezo = static_cast<const T*>(C->fo[2]);
-    const T* Hxo = static_cast<const T*>(C->fo[3]);
-    const T* Hyo = static_cast<const T*>(C->fo[4]);
-    const T* Hzo = static_cast<const T*>(C->fo[5]);
-    const long long pb = plane(p), pn = plane(p + 1), pm = plane(p - 2);
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int off = fpos[u] >= 0 ? foff[u] : -1;
-      S.ex[u] = ldz(Exo, pb, off, g.zidx);
-      S.ey[u] = ldz(Eyo, pn, off, g.zidx);
-      S.ez[u] = ldz(Ezo, pn, off, g.zidx);
-    }
-    S.hx = ldz(Hxo, pb, ho_m, g.zidx);
-    S.hy = ldz(Hyo, pb, ho_m, g.zidx);
-    S.hz = ldz(Hzo, pb, ho_m, g.zidx);
-    if (ty == 0) {  // wave-uniform: the halo wave
-      S.gx = ldz(Hxo, pb, ho_h, g.zidx);
-      S.gy = ldz(Hyo, pb, ho_h, g.zidx);
-      S.gz = ldz(Hzo, pb, ho_h, g.zidx);
-    }
-    S.cls = ldz(clsp, clsg ? pm : 0, clsg ? ho_e : 0, czidx);
-  };
-  auto ring_direct = [&](int p) {  // Ey / Ez tiles of plane p (prologue)
-    const long long pb = plane(p);
-    const T* Eyo = static_cast<const T*>(C0->fo[1]);
-    const T* Ezo = static_cast<const T*>(C0->fo[2]);
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      if (fpos[u] < 0) continue;
-      rEy[slot<5>(p)][fpos[u]] = cv(ldz(Eyo, pb, foff[u], g.zidx), iey);
-      rEz[slot<5>(p)][fpos[u]] = cv(ldz(Ezo, pb, foff[u], g.zidx), iez);
-    }
-  };
-
-  // per-column CPML state (z slab per column; y slab per row in GY launches)
-  const auto& zp = C0->zp;
-  const bool pm_on = zp.on;
-  const int xlo = zp.plo[0], xhi = zp.phi[0];
-  const unsigned Px = zp.plo[0] + zp.phi[0], Py = zp.plo[1] + zp.phi[1], Pz = zp.plo[2] + zp.phi[2];
-  const int zsm = (pm_on && hv_m) ? slab_index(kmm, g.nz, zp.plo[2], zp.phi[2]) : -1;
-  const int zsh = (pm_on && hv_h) ? slab_index(kmh, g.nz, zp.plo[2], zp.phi[2]) : -1;
-  const int ysl = (GY && pm_on && jm >= 0) ? slab_index(jm, g.ny, zp.plo[1], zp.phi[1]) : -1;
-  const int yslh = (GY && pm_on && hv_h) ? slab_index(jmh, g.ny, zp.plo[1], zp.phi[1]) : -1;
-  const auto& pc = C0->pmo;  // coefficient tables
-  const float bzm = zsm >= 0 ? gl(pc.bH[2])[kmm] : 0.f, czm = zsm >= 0 ? gl(pc.cH[2])[kmm] : 0.f;
-  const float bzh = zsh >= 0 ? gl(pc.bH[2])[kmh] : 0.f, czh = zsh >= 0 ? gl(pc.cH[2])[kmh] : 0.f;
-  const float bze = zsm >= 0 ? gl(pc.bE[2])[kmm] : 0.f, cze = zsm >= 0 ? gl(pc.cE[2])[kmm] : 0.f;
-  const float byh = ysl >= 0 ? gl(pc.bH[1])[jm] : 0.f, cyh = ysl >= 0 ? gl(pc.cH[1])[jm] : 0.f;
-  const float byhh = yslh >= 0 ? gl(pc.bH[1])[jmh] : 0.f, cyhh = yslh >= 0 ? gl(pc.cH[1])[jmh] : 0.f;
-  const float bye = ysl >= 0 ? gl(pc.bE[1])[jm] : 0.f, cye = ysl >= 0 ? gl(pc.cE[1])[jm] : 0.f;
-  const auto& mc = C0->mt;
-  const bool design_col = mc.d_on && jh >= mc.d_j0 && jh < mc.d_j0 + mc.d_nj && khm >= mc.d_k0 &&
-                          khm < mc.d_k0 + mc.d_nk;
-  const int di0 = mc.d_i0, di1 = mc.d_i0 + mc.d_ni;
-  const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
-  const int p0 = t.i0 - 1, pend = t.i1 + 1;  // H planes p0 .. pend
-
-  PreT<T> S[PF];
-#pragma unroll
-  for (int u = 0; u < PF; u++) {
-    S[u] = PreT<T>{};
-  }
-  ring_direct(p0 - 2);
-  ring_direct(p0 - 1);
-  ring_direct(p0);
-#pragma unroll
-  for (int u = 0; u < PF; u++) fetch(S[u], min(p0 + u, pend), C0);
-
-  // H at tile row `row`, column c of plane p (grid row jr, column km) from
-  // the E rings; CPML from the old psi (written new only when `own`); returns
-  // the three new components.
-  auto h_at = [&](const ColdK* C, int p, int row, int c, int ip, int jr, int km, int zs, float bz, float cz,
-                  int ys, float by, float cy, bool valid, bool own, float hx0, float hy0, float hz0, float pz0,
-                  float pz1, float py0, float py1, float& hx, float& hy, float& hz) {
-    if (!valid) {
-      hx = hy = hz = 0.f;
-      return;
-    }
-    const int e = (row + 2) * EC + (c + 2);  // this H position in the E tile
-    const float* tEx = rEx[slot<4>(p)];
-    const float* tEy = rEy[slot<5>(p)];
-    const float* tEz = rEz[slot<5>(p)];
-    const float ex0 = tEx[e];
-    float dyEz = rd1 * d24(tEz[e - 2 * EC], tEz[e - EC], tEz[e], tEz[e + EC]);
-    float dyEx = rd1 * d24(tEx[e - 2 * EC], tEx[e - EC], ex0, tEx[e + EC]);
-    float dzEy = rd2 * d24(tEy[e - 2], tEy[e - 1], tEy[e], tEy[e + 1]);
-    float dzEx = rd2 * d24(tEx[e - 2], tEx[e - 1], ex0, tEx[e + 1]);
-    float dxEz = rd0 * d24(rEz[slot<5>(p - 2)][e], rEz[slot<5>(p - 1)][e], tEz[e], rEz[slot<5>(p + 1)][e]);
-    float dxEy = rd0 * d24(rEy[slot<5>(p - 2)][e], rEy[slot<5>(p - 1)][e], tEy[e], rEy[slot<5>(p + 1)][e]);
-    if (zs >= 0) {  // z axis: slot 0 = Hx (D_z Ey), slot 1 = Hy (D_z Ex)
-      const unsigned pq = psi_index(g, Pz, 2, zs, ip, jr, km);
-      cpml_d(pz0, C->zp.hN[0], pq, bz, cz, dzEy, own);
-      cpml_d(pz1, C->zp.hN[1], pq, bz, cz, dzEx, own);
-    }
-    if (GX && pm_on) {
-      const int xs = slab_index(ip, g.nx, xlo, xhi);
-      if (xs >= 0) {  // x axis (read in place: a few planes): slot 0 = Hy (D_x Ez), slot 1 = Hz (D_x Ey)
-        const float bb = gl(C->pmo.bH[0])[ip], cc = gl(C->pmo.cH[0])[ip];
-        const unsigned pq = psi_index(g, Px, 0, xs, ip, jr, km);
-        cpml_d(gl(C->pmo.psiH[0][0])[pq], C->pmn.psiH[0][0], pq, bb, cc, dxEz, own);
-        cpml_d(gl(C->pmo.psiH[0][1])[pq], C->pmn.psiH[0][1], pq, bb, cc, dxEy, own);
-      }
-    }
-    if (GY && ys >= 0) {  // y axis: slot 0 = Hz (D_y Ex), slot 1 = Hx (D_y Ez)
-      const unsigned pq = psi_index(g, Py, 1, ys, ip, jr, km);
-      cpml_d(py0, C->pmn.psiH[1][0], pq, by, cy, dyEx, own);
-      cpml_d(py1, C->pmn.psiH[1][1], pq, by, cy, dyEz, own);
-    }
-    hx = hx0 - dt * (dyEz - dzEy);
-    hy = hy0 - dt * (dzEx - dxEz);
-    hz = hz0 - dt * (dxEy - dyEx);
-  };
-
-  auto step = [&](int p, PreT<T>& Q) {
-    const int m = p - 2;
-    // the cold operands are re-read on every plane, never held across planes
-    const TorreCold* Cg = cold;
-    asm volatile("" : "+s"(Cg));
-    const ColdK* C = (const ColdK*)Cg;
-    // 1) publish Ex of plane p, Ey / Ez of plane p+1, the old H, the E class
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      if (fpos[u] < 0) continue;
-      rEx[slot<4>(p)][fpos[u]] = cv(Q.ex[u], iex);
-      rEy[slot<5>(p + 1)][fpos[u]] = cv(Q.ey[u], iey);
-      rEz[slot<5>(p + 1)][fpos[u]] = cv(Q.ez[u], iez);
-    }
-    sHo[0][ty][tx + 1] = cv(Q.hx, ihx);
-    sHo[1][ty][tx + 1] = cv(Q.hy, ihy);
-    sHo[2][ty][tx + 1] = cv(Q.hz, ihz);
-    if (hal) {
-      sHo[0][hr][chh] = cv(Q.gx, ihx);
-      sHo[1][hr][chh] = cv(Q.gy, ihy);
-      sHo[2][hr][chh] = cv(Q.gz, ihz);
-    }
-    sCl[ty][tx] = Q.cls;
-    // 2) psi and design operands of this plane, before the prefetch
-    const long long pb = plane(p);
-    const int ip = GX ? nbr(p, g.nx, g.per[0]) : p;
-    const bool hplane = pb >= 0;
-    const int im = GX ? nbr(m, g.nx, g.per[0]) : m;
-    const bool edo = eown && m >= t.i0 && m < t.i1;
-    float pzm0 = 0.f, pzm1 = 0.f, pzh0 = 0.f, pzh1 = 0.f, pym0 = 0.f, pym1 = 0.f, pyh0 = 0.f, pyh1 = 0.f;
-    float pze0 = 0.f, pze1 = 0.f, pye0 = 0.f, pye1 = 0.f, dv = 0.f;
-    if (hplane && zsm >= 0) {
-      const unsigned pq = psi_index(g, Pz, 2, zsm, ip, jm, kmm);
-      pzm0 = gl(C->zp.hO[0])[pq];
-      pzm1 = gl(C->zp.hO[1])[pq];
-    }
-    if (hplane && zsh >= 0) {
-      const unsigned pq = psi_index(g, Pz, 2, zsh, ip, jmh, kmh);
-      pzh0 = gl(C->zp.hO[0])[pq];
-      pzh1 = gl(C->zp.hO[1])[pq];
-    }
-    if (GY && hplane && ysl >= 0 && hv_m) {
-      const unsigned pq = psi_index(g, Py, 1, ysl, ip, jm, kmm);
-      pym0 = gl(C->pmo.psiH[1][0])[pq];
-      pym1 = gl(C->pmo.psiH[1][1])[pq];
-    }
-    if (GY && hplane && yslh >= 0) {
-      const unsigned qh = psi_index(g, Py, 1, yslh, ip, jmh, kmh);
-      pyh0 = gl(C->pmo.psiH[1][0])[qh];
-      pyh1 = gl(C->pmo.psiH[1][1])[qh];
-    }
-    if (edo && zsm >= 0) {
-      const unsigned pq = psi_index(g, Pz, 2, zsm, im, jm, kmm);
-      pze0 = gl(C->zp.eO[0])[pq];
-      pze1 = gl(C->zp.eO[1])[pq];
-    }
-    if (GY && edo && ysl >= 0) {
-      const unsigned pq = psi_index(g, Py, 1, ysl, im, jm, kmm);
-      pye0 = gl(C->pmo.psiE[1][0])[pq];
-      pye1 = gl(C->pmo.psiE[1][1])[pq];
-    }
-    const bool over = edo && design_col && m >= di0 && m < di1;
-    if (over) {
-      const auto& M = C->mt;
-      dv = gl(M.d_inv_eps)[((long long)(m - M.d_i0) * M.d_nj + (jh - M.d_j0)) * M.d_nk + (khm - M.d_k0)];
-    }
-    // 3) prefetch plane p+PF (clamped: unconditional)
-    fetch(Q, min(p + PF, pend), C);
-    lds_barrier();
-
-    // 4) H at plane p: main column, then the halo column (lanes 0..2)
-    const bool hown = hown_c && p >= t.i0 && p < t.i1;
-    float hx, hy, hz;
-    h_at(C, p, ty, tx + 1, ip, jm, kmm, zsm, bzm, czm, ysl, byh, cyh, hplane && hv_m, hown, sHo[0][ty][tx + 1],
-         sHo[1][ty][tx + 1], sHo[2][ty][tx + 1], pzm0, pzm1, pym0, pym1, hx, hy, hz);
-    const int hs = slot<5>(p);
-    rH[hs][0][ty][tx + 1] = hx;
-    rH[hs][1][ty][tx + 1] = hy;
-    rH[hs][2][ty][tx + 1] = hz;
-    if (hown) {
-      const long long q = pb + ho_m;
-      st(static_cast<T*>(C->fn[3]), q, hx, sc.sh[0]);
-      st(static_cast<T*>(C->fn[4]), q, hy, sc.sh[1]);
-      st(static_cast<T*>(C->fn[5]), q, hz, sc.sh[2]);
-    }
-    if (hal) {
-      float gx, gy, gz;
-      h_at(C, p, hr, chh, ip, jmh, kmh, zsh, bzh, czh, yslh, byhh, cyhh, hplane && hv_h, false, sHo[0][hr][chh],
-           sHo[1][hr][chh], sHo[2][hr][chh], pzh0, pzh1, pyh0, pyh1, gx, gy, gz);
-      rH[hs][0][hr][chh] = gx;
-      rH[hs][1][hr][chh] = gy;
-      rH[hs][2][hr][chh] = gz;
-    }
-
-    // 5) E at plane m = p-2 (owned cells), from the H ring
-    if (edo) {
-      const int c = tx + 1;  // this E cell's column in the H tile
-      const float(*h0)[TR][HC] = rH[slot<5>(m)];
-      const float hx0 = h0[0][ty][c], hy0 = h0[1][ty][c], hz0 = h0[2][ty][c];
-      float dyHz = rd1 * d24(h0[2][ty - 1][c], hz0, h0[2][ty + 1][c], h0[2][ty + 2][c]);
-      float dyHx = rd1 * d24(h0[0][ty - 1][c], hx0, h0[0][ty + 1][c], h0[0][ty + 2][c]);
-      float dzHy = rd2 * d24(h0[1][ty][c - 1], hy0, h0[1][ty][c + 1], h0[1][ty][c + 2]);
-      float dzHx = rd2 * d24(h0[0][ty][c - 1], hx0, h0[0][ty][c + 1], h0[0][ty][c + 2]);
-      float dxHz = rd0 * d24(rH[slot<5>(m - 1)][2][ty][c], hz0, rH[slot<5>(m + 1)][2][ty][c], rH[slot<5>(m + 2)][2][ty][c]);
-      float dxHy = rd0 * d24(rH[slot<5>(m - 1)][1][ty][c], hy0, rH[slot<5>(m + 1)][1][ty][c], rH[slot<5>(m + 2)][1][ty][c]);
-      if (zsm >= 0) {  // z axis: slot 0 = Ex (D_z Hy), slot 1 = Ey (D_z Hx)
-        const unsigned pq = psi_index(g, Pz, 2, zsm, im, jm, kmm);
-        cpml_d(pze0, C->zp.eN[0], pq, bze, cze, dzHy, true);
-        cpml_d(pze1, C->zp.eN[1], pq, bze, cze, dzHx, true);
-      }
-      if (GX && pm_on) {
-        const int xs = slab_index(im, g.nx, xlo, xhi);
-        if (xs >= 0) {  // x axis: slot 0 = Ey (D_x Hz), slot 1 = Ez (D_x Hy)
-          const float bb = gl(C->pmo.bE[0])[im], cc = gl(C->pmo.cE[0])[im];
-          const unsigned pq = psi_index(g, Px, 0, xs, im, jm, kmm);
-          cpml_d(gl(C->pmo.psiE[0][0])[pq], C->pmn.psiE[0][0], pq, bb, cc, dxHz, true);
-          cpml_d(gl(C->pmo.psiE[0][1])[pq], C->pmn.psiE[0][1], pq, bb, cc, dxHy, true);
-        }
-      }
-      if (GY && ysl >= 0) {  // y axis: slot 0 = Ez (D_y Hx), slot 1 = Ex (D_y Hz)
-        const unsigned pq = psi_index(g, Py, 1, ysl, im, jm, kmm);
-        cpml_d(pye0, C->pmn.psiE[1][0], pq, bye, cye, dyHx, true);
-        cpml_d(pye1, C->pmn.psiE[1][1], pq, bye, cye, dyHz, true);
-      }
-      const float cx = dyHz - dzHy, cy = dzHx - dxHz, cz = dxHy - dyHx;
-      const long long q = (long long)im * g.sx + ho_m;
-      const ClassEntry& ce = tbl[sCl[ty][tx]];
-      float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
-      if (over) ie0 = ie1 = ie2 = dv;
-      if (ce.special && !over) {  // finished by k_ade (old E -> new E)
-        const auto& A = C->ade;
-        const long long bq = (long long)(m - A.i0) * A.bsx + (long long)(jh - A.j0) * A.bsy + (khm - A.k0);
-        gl(A.curl[0])[bq] = cx;
-        gl(A.curl[1])[bq] = cy;
-        gl(A.curl[2])[bq] = cz;
-      } else {
-        const int e = (ty + 2) * EC + (tx + 3);  // old E of this cell in the rings (plane m)
-        st(static_cast<T*>(C->fn[0]), q, rEx[slot<4>(m)][e] + dt * ie0 * cx, sc.se[0]);
-        st(static_cast<T*>(C->fn[1]), q, rEy[slot<5>(m)][e] + dt * ie1 * cy, sc.se[1]);
-        st(static_cast<T*>(C->fn[2]), q, rEz[slot<5>(m)][e] + dt * ie2 * cz, sc.se[2]);
-      }
-    }
-  };
-
-  int p = p0;
-  for (; p + PF - 1 <= pend; p += PF) {
-#pragma unroll
-    for (int u = 0; u < PF; u++) step(p + u, S[u]);
-  }
-#pragma unroll
-  for (int u = 0; u < PF - 1; u++)
-    if (p + u <= pend) step(p + u, S[u]);
-}
-
 // ============================================ fused step with x register queues
-// The same step as k_torre, the same tile (H on 16 x 67 positions, E on the
-// owned 13 x 64) and the same x march with E two planes behind H, but the x
-// stencils come from registers: every thread keeps, for its own (y, z)
+// The x stencils come from registers: every thread keeps, for its own (y, z)
 // column, the four planes of E^n that the H update's x derivative reads and
 // the four planes of H^{n+1/2} that the E update's reads (the wave-0 lanes
 // that evaluate the halo H columns keep their own E queue).  LDS then holds

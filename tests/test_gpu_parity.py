@@ -87,9 +87,10 @@ def test_vacuum_dipole_small_cpml_sensors():
     assert rel_l2([gs], [os_]) <= TOL
 
 
-def test_vacuum_dipole_full_size_1000_steps():
+def test_vacuum_dipole_full_size_1000_steps(sweep):
     """The 128^3 workload (10-cell CPML, soft Ez dipole, 8 seeded sensors,
-    1000 steps): FP32 HIP vs FP64 oracle."""
+    1000 steps): FP32 HIP vs FP64 oracle, and FP16 storage (scale 2^10) vs
+    the same oracle, held to 1e-2."""
     w = configs.vacuum_dipole()
     g, o = run_pair(w)
     err = check_fields(g, o)
@@ -97,6 +98,17 @@ def test_vacuum_dipole_full_size_1000_steps():
     print(f"sensor rel-L2 {serr:.3e}")
     assert serr <= TOL
     assert err <= TOL
+    if sweep == abi.SWEEP_TWOPASS:
+        w16 = configs.vacuum_dipole()
+        w16.cfg.storage = abi.STORE_FP16
+        w16.cfg.scale_e_log2[:] = [10] * 3
+        w16.cfg.scale_h_log2[:] = [10] * 3
+        h = configs.apply(w16, solver(w16.cfg))
+        h.step(w16.steps)
+        e16 = rel_l2(h.fields(), o.fields())
+        s16 = rel_l2([h.probes()], [o.probes()])
+        print(f"fp16 vs fp64: fields {e16:.3e} sensors {s16:.3e}")
+        assert e16 < 1e-2 and s16 < 1e-2
 
 
 def test_plane_wave_sphere_small():
