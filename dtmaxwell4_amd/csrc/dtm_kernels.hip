@@ -1046,14 +1046,16 @@ template <int V> struct ClsWord;
 template <> struct ClsWord<1> { using T = uint8_t; };
 template <> struct ClsWord<2> { using T = unsigned short; };
 
-template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false>
+// MAT: the grid may hold dispersive / gyrotropic cells or a design box; the
+// variants without (MAT false) carry none of their branches and pointers.
+template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false, bool MAT = true>
 __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
                                                          Ade ade, MarchSet ms, const float* fold) {
   using T = typename Raw<STORE>::T;
   using P = typename Vec<STORE, V>::P;
   using CL = typename ClsWord<V>::T;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
-  constexpr bool DES = V == 1;  // design-box support (its launches use one cell per lane)
+  constexpr bool DES = MAT && V == 1;  // design-box support (its launches use one cell per lane)
   constexpr int BZV = BZ * V, LZ = BZV + 3, MBY = ROWS;
   __shared__ ClassEntry tbl[kMaxClasses];
   __shared__ float sHx[2][MBY + 3][LZ];  // [row j-j0+1][col k-k0+1], plane i
@@ -1327,7 +1329,7 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
         const ClassEntry& ce = tbl[cls];
         float ie0 = ce.inv_eps[0], ie1 = ce.inv_eps[1], ie2 = ce.inv_eps[2];
         if (over[q]) ie0 = ie1 = ie2 = dv[q];
-        if (ce.special && !over[q]) {  // finished by k_ade; its E word keeps the old value here
+        if (MAT && ce.special && !over[q]) {  // finished by k_ade; its E word keeps the old value here
           const long long bq =
               (long long)(i - ade.i0) * ade.bsx + (long long)(j - ade.j0) * ade.bsy + (k + q - ade.k0);
           ade.curl[0][bq] = crx;
@@ -1587,30 +1589,30 @@ static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpm
   check_launch("k_update_h");
 }
 
-template <int STORE, int V, bool SP, bool XZ, bool TF>
+template <int STORE, int V, bool SP, bool XZ, bool TF, bool MAT = true>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                   const Ade& ade, const Region* regs, int nreg, const float* fold, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   int seg8 = 0, seg16 = 0;
   if constexpr (V == 1) {
-    static const int slots8 = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
-    static const int slots16 = resident_blocks(k_update_e<STORE, 3, V, SP, 16, XZ, TF>, BZ * 16);
+    static const int slots8 = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT>, BZ * 8);
+    static const int slots16 = resident_blocks(k_update_e<STORE, 3, V, SP, 16, XZ, TF, MAT>, BZ * 16);
     if (use_16_rows(slots8, slots16, regs, nreg, BZ, seg8, seg16)) {
       MarchSet rg = make_march(regs, nreg, seg16, BZ, 16);
       if (rg.total <= 0) return;
-      hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 16, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
-                         mt, ade, rg, fold);
+      hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 16, XZ, TF, MAT>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g,
+                         f, sc, pm, mt, ade, rg, fold);
       check_launch("k_update_e");
       return;
     }
   } else {
-    static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
+    static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT>, BZ * 8);
     seg8 = march_seg(regs, nreg, slots, BZ * V, 8);
   }
   MarchSet rg = make_march(regs, nreg, seg8, BZ * V, 8);
   if (rg.total <= 0) return;
-  hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 8, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, mt,
-                     ade, rg, fold);
+  hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc,
+                     pm, mt, ade, rg, fold);
   check_launch("k_update_e");
 }
 
@@ -1656,19 +1658,23 @@ static void update_e_go(int store, const Grid& g, const void* fields, const Scal
                         hipStream_t st) {
   const bool xz = store == 1 && sc.xz.on;
   const bool pairs = pair_cells(store, g, &mt) && even_k(regs, nreg);
+  const bool mat = ade.on || mt.d_on;  // else the variants without the dispersive / design branches
   if (pm.split) {
     if (store == 0) upd_e<0, 1, true, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
     else if (xz) upd_e<1, 1, true, true, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
     else upd_e<1, 1, true, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
   } else if (store == 0) {
-    upd_e<0, 1, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    if (mat) upd_e<0, 1, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else upd_e<0, 1, false, false, TF, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
   } else if (xz) {
     if (pairs) upd_e<1, 2, false, true, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
     else upd_e<1, 1, false, true, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
   } else if (pairs) {
-    upd_e<1, 2, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    if (mat) upd_e<1, 2, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else upd_e<1, 2, false, false, TF, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
   } else {
-    upd_e<1, 1, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    if (mat) upd_e<1, 1, false, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else upd_e<1, 1, false, false, TF, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
   }
 }
 
