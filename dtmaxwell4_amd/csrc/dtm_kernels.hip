@@ -327,11 +327,12 @@ __device__ __forceinline__ void cpml_step(float* psi, long long pq, float old, f
 
 // ===================================================== marching sweeps
 // A block owns a tile of 64 (z) x ROWS (y) columns and marches along x over
-// a segment of planes.  Per thread, the x-neighbours of its column live in a
-// 4-deep register queue (the tilted space-time march of the reference's
-// torre, genAlg.py:420-499, with the rag shuffles of copyrags.cuh replaced
-// by a register queue); the y/z neighbours of the current plane come from a
-// double-buffered LDS tile with a 3-cell halo.  One barrier per plane.
+// a segment of planes (the tilted space-time march of the reference's torre,
+// genAlg.py:420-499, with the rag shuffles of copyrags.cuh replaced by an
+// LDS ring): the x stencil reads a 5-plane ring of the two transverse
+// components, the y/z stencils a double-buffered LDS tile of the current
+// plane with a 3-cell halo, and operands are prefetched PF planes ahead in
+// registers.  One barrier per plane.
 // y rows per block (ROWS): 8, or 16 with one cell per lane when the launch
 // then fills the chip as well (see upd_h).  At 512x256x256 the FP32 E sweep
 // ran 7% faster with 16 rows (the H sweep 1%); the paired FP16 sweeps ran 8%
