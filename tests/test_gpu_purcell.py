@@ -1,15 +1,17 @@
 """Purcell factor of the dispersive dipole run (Drude + Lorentz sphere, the
-reference's DipoleTFSF with the scattered field inside the box): FP32 HIP
-path against the FP64 oracle to the stated Purcell bound 1e-3 (the scattered
-field at the dipole is far below the grid's field scale, so its FP32 error is
-well above the 1e-5 field parity), and FP16 storage against FP32.  Error
-measure: max_w |P_a(w) - P_b(w)| / max_w |P_b(w)|.  FP16 stores the total
-field just outside the small TF/SF box, where the dipole's near field is
-largest, and the box corrections subtract the FP64 incident field from it:
-the stored rounding error lands on the scattered field the Purcell factor is
-made of, every step, and it accumulates over the run (measured here: 2e-2
-after 600 steps, 0.27 after 1500).  The FP16 check is therefore a sanity
-bound (finite, same sign structure); bench.py reports the full-size error."""
+reference's DipoleTFSF with the scattered field inside the box), against the
+FP64 oracle's Purcell factor P64 of the same run.  Error measure:
+max_w |P(w) - P64(w)| / max_w |P64(w)|.
+
+- FP32 storage: the stated Purcell bound 1e-3 (the scattered field at the
+  dipole is far below the grid's field scale, so its FP32 error is well
+  above the 1e-5 field parity).
+- FP16 storage: it stores the total field just outside the small TF/SF box,
+  where the dipole's near field is largest, and the box corrections subtract
+  the FP64 incident field from it, so the rounding lands on the scattered
+  field P is made of, every step (measured here 6.7e-3; bound 2e-2).  With
+  the FP32 zone around the box (configs.dipole_zone) the bound is 1e-3
+  (measured 6e-5)."""
 import numpy as np
 import pytest
 
@@ -34,7 +36,7 @@ def small(storage=abi.STORE_FP32):
                                          cells_per_lambda=20, storage=storage)
 
 
-def test_purcell_fp32_matches_fp64_oracle_and_fp16_tracks_fp32():
+def test_purcell_fp32_and_fp16_against_the_fp64_oracle():
     from dtmaxwell4_amd.solver import Solver
     w32 = small()
     g = configs.apply(w32, Solver(w32.cfg))
@@ -47,13 +49,21 @@ def test_purcell_fp32_matches_fp64_oracle_and_fp16_tracks_fp32():
     print("P range", P64.min(), P64.max(), "fp32 vs fp64", e32)
     assert np.ptp(P64) > 1e-3          # the sphere changes P measurably
     assert e32 < 1e-3
-    w16 = small(abi.STORE_FP16)
-    h = configs.apply(w16, Solver(w16.cfg))
-    h.step(w16.steps)
-    P16 = purcell_curve(h, w16)
-    e16 = perr(P16, P32)
-    print("fp16 vs fp32", e16)
-    assert np.isfinite(P16).all() and e16 < 1.0
+    curves = {}
+    for zone in (False, True):
+        w16 = small(abi.STORE_FP16)
+        h = Solver(w16.cfg)
+        if zone:
+            h.set_fp32_zone(*configs.dipole_zone(w16))
+        configs.apply(w16, h)
+        h.step(w16.steps)
+        curves[zone] = purcell_curve(h, w16)
+        h.close()
+    e16, e16z = perr(curves[False], P64), perr(curves[True], P64)
+    print("fp16 vs fp64", e16, "fp16 with the FP32 zone vs fp64", e16z)
+    assert np.isfinite(curves[False]).all()
+    assert e16 < 2e-2
+    assert e16z <= 1e-3
 
 
 def test_dipole_box_too_small_for_the_singular_field_is_rejected():
