@@ -13,6 +13,7 @@ BC_ZERO, BC_PERIODIC = 0, 1
 STORE_FP32, STORE_FP16 = 0, 1
 SWEEP_TWOPASS, SWEEP_FUSED = 0, 1
 PML_CPML, PML_SPLIT = 0, 1
+PML_PROFILE_POLY, PML_PROFILE_REFERENCE = 0, 1
 TFSF_NONE, TFSF_DIPOLE, TFSF_PLANE = 0, 1, 2
 WAVE_RAISED_COS, WAVE_GAUSS_SINE, WAVE_GAUSS = 0, 1, 2
 MAX_MATS, MAX_POLES, MAX_SOFT = 64, 4, 16
@@ -97,7 +98,8 @@ class Config(C.Structure):
                 ("storage", C.c_int), ("scale_e_log2", C.c_int * 3),
                 ("scale_h_log2", C.c_int * 3), ("nmat", C.c_int),
                 ("mat", Material * MAX_MATS), ("tfsf", Tfsf), ("ntime", C.c_int),
-                ("nsoft", C.c_int), ("soft", SoftSource * MAX_SOFT), ("sweep", C.c_int)]
+                ("nsoft", C.c_int), ("soft", SoftSource * MAX_SOFT), ("sweep", C.c_int),
+                ("pml_profile", C.c_int)]
 
 
 def default_config(nx, ny, nz, dx=0.005, dy=0.005, dz=0.005, dt=0.002):

@@ -7,9 +7,13 @@
 //    GridNx/GridNy/GridNz are the true x/y/z extents (the reference's
 //    Python-visible GridNy/GridNz are swapped, py_consts.h:10-12 vs
 //    params.h:44-49);
-//  * the absorbing layer is CPML, Npml* cells per side, on every face but
-//    the low-z one (non-absorbing there, as in the reference, DTmxw.cu:262;
-//    the low-z face is a zero boundary inside the silver film);
+//  * the absorbing layer is the reference's split-field layer with its sigma
+//    profile (DTM_PML_PROFILE_REFERENCE: the x grading of DTmxw.cu:219-253
+//    on every face), Npml* cells per side, on every face but the low-z one
+//    (non-absorbing there, as in the reference, DTmxw.cu:262; the low-z face
+//    is a zero boundary inside the silver film); the reference's y and z
+//    tables call the same sigma function with other index offsets
+//    (DTmxw.cu:257, 261), which is not reproduced;
 //  * no OpenGL viewer: without --test, _main reports that and returns -1.
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -268,6 +272,8 @@ int _main(int argc, char** argv) {
   c.npml[4] = 0;
   c.npml[5] = Npmlz;
   c.pml_vmax = shotpoint.V_max > 0 ? shotpoint.V_max : 1.0;
+  c.pml_kind = DTM_PML_SPLIT;  // genAlg.py:117-148
+  c.pml_profile = DTM_PML_PROFILE_REFERENCE;
   c.ntime = Ntime;
   set_materials(c);
   // DipoleTFSF around the shot point, scattered field inside (signal.hpp:135-160)

@@ -229,19 +229,24 @@ void dtm_config_defaults(dtm_config* c, int nx, int ny, int nz) {
 
 }  // extern "C"
 
-// sigma at half-coordinate h (same grading as DTmxw.cu:220-249 with the
-// standard CPML depth measured from the inner face of the slab).
-static double pml_sigma(int h, int n, int plo, int phi, double d, double m, double R, double vmax) {
-  if (plo > 0 && h < 2 * plo) {
-    const double L = plo * d;
-    const double smax = vmax * log(1.0 / R) * (m + 1) / (2 * L);
-    return smax * pow((double)(2 * plo - h) / (2.0 * plo), m);
+// sigma u half-cells from the outer face of a layer of p cells.  POLY: the
+// grading of DTmxw.cu:220-249 over the whole layer; REFERENCE: PMLgamma_func
+// (DTmxw.cu:219-228) as setPMLcoeffs indexes it along x (DTmxw.cu:251-253),
+// flat over the outer 3 half-cells, graded over N = 2p - 6, zero over the
+// inner 3.
+static double layer_sigma(int u, int p, double d, double m, double R, double vmax, int profile) {
+  if (profile == DTM_PML_PROFILE_REFERENCE) {
+    const double N = 2.0 * p - 6, a = std::max(u - 3, 0);
+    if (a >= N) return 0.0;
+    return vmax * log(1.0 / R) * (m + 1) / (N * d) * pow(1.0 - a / N, m);
   }
-  if (phi > 0 && h > 2 * (n - phi)) {
-    const double L = phi * d;
-    const double smax = vmax * log(1.0 / R) * (m + 1) / (2 * L);
-    return smax * pow((double)(h - 2 * (n - phi)) / (2.0 * phi), m);
-  }
+  const double smax = vmax * log(1.0 / R) * (m + 1) / (2.0 * p * d);
+  return smax * pow((2.0 * p - u) / (2.0 * p), m);
+}
+// sigma at half-coordinate h of an axis of n cells with layers of plo / phi cells
+static double pml_sigma(int h, int n, int plo, int phi, double d, double m, double R, double vmax, int profile) {
+  if (plo > 0 && h < 2 * plo) return layer_sigma(h, plo, d, m, R, vmax, profile);
+  if (phi > 0 && h > 2 * (n - phi)) return layer_sigma(2 * n - h, phi, d, m, R, vmax, profile);
   return 0.0;
 }
 
@@ -520,8 +525,10 @@ static int setup_cpml(dtm_solver* s) {
   for (int a = 0; a < 3; a++) {
     std::vector<float> bE(n[a]), cE(n[a]), bH(n[a]), cH(n[a]);
     for (int v = 0; v < n[a]; v++) {
-      const double se = pml_sigma(2 * v + 1, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax);
-      const double sh = pml_sigma(2 * v, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax);
+      const double se =
+          pml_sigma(2 * v + 1, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax, c.pml_profile);
+      const double sh =
+          pml_sigma(2 * v, n[a], p.plo[a], p.phi[a], d[a], c.pml_order, c.pml_R, c.pml_vmax, c.pml_profile);
       if (p.split) {  // setPMLcoeffs (DTmxw.cu:250-265)
         const double k2e = 1.0 / (1.0 + 0.5 * c.dt * se), k2h = 1.0 / (1.0 + 0.5 * c.dt * sh);
         bE[v] = (float)(2.0 * k2e - 1.0);
@@ -792,6 +799,12 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   if (c.storage != DTM_STORE_FP32 && c.storage != DTM_STORE_FP16) return fail(DTM_ERR_ARG, "bad storage");
   if (c.sweep != DTM_SWEEP_TWOPASS && c.sweep != DTM_SWEEP_FUSED) return fail(DTM_ERR_ARG, "bad sweep");
   if (c.pml_kind != DTM_PML_CPML && c.pml_kind != DTM_PML_SPLIT) return fail(DTM_ERR_ARG, "bad pml_kind");
+  if (c.pml_profile != DTM_PML_PROFILE_POLY && c.pml_profile != DTM_PML_PROFILE_REFERENCE)
+    return fail(DTM_ERR_ARG, "bad pml_profile");
+  if (c.pml_profile == DTM_PML_PROFILE_REFERENCE)
+    for (int q = 0; q < 6; q++)
+      if (c.npml[q] > 0 && c.npml[q] < 4)
+        return fail(DTM_ERR_ARG, "the reference sigma profile needs layers of at least 4 cells");
   if (c.pml_kind == DTM_PML_SPLIT && c.sweep == DTM_SWEEP_FUSED)
     return fail(DTM_ERR_ARG, "the split-field layer runs with the two-pass sweeps");
   if (c.sweep == DTM_SWEEP_FUSED &&

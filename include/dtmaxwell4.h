@@ -55,6 +55,15 @@ enum { DTM_WAVE_RAISED_COS = 0, DTM_WAVE_GAUSS_SINE = 1, DTM_WAVE_GAUSS = 2 };
  * as part = k1 part +- k2 dif with k2 = 1/(1 + sigma dt/2), k1 = 2 k2 - 1
  * (DTmxw.cu:250-265).  Both use the same sigma grading. */
 enum { DTM_PML_CPML = 0, DTM_PML_SPLIT = 1 };
+/* sigma grading across a layer of p cells (2p half-cells), u half-cells from
+ * its outer face:
+ *  POLY       sigma_max ((2p - u) / 2p)^m, sigma_max = V ln(1/R) (m+1) / (2 p d);
+ *  REFERENCE  PMLgamma_func (DTmxw.cu:219-228) with the index map of
+ *             setPMLcoeffs for x (DTmxw.cu:251-253): sigma_max over the outer
+ *             3 half-cells, sigma_max (1 - (u-3)/N)^m over N = 2p - 6
+ *             half-cells, zero over the inner 3; sigma_max = V ln(1/R) (m+1)
+ *             / (N d).  Needs p >= 4 on every face with a layer. */
+enum { DTM_PML_PROFILE_POLY = 0, DTM_PML_PROFILE_REFERENCE = 1 };
 enum {
   DTM_OK = 0,
   DTM_ERR_ARG = -1,      /* invalid argument / configuration */
@@ -182,6 +191,7 @@ typedef struct dtm_config {
   int nsoft;
   dtm_soft_source soft[DTM_MAX_SOFT];
   int sweep;              /* DTM_SWEEP_*: time-step schedule (same results) */
+  int pml_profile;        /* DTM_PML_PROFILE_*: sigma grading across the layer */
 } dtm_config;
 
 /* Fill a config with the reference's defaults (dx=dy=dz=0.005, dt=0.002,

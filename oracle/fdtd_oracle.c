@@ -25,6 +25,10 @@
  *   - absorbing layer: CPML with the reference's sigma grading
  *     (DTmxw.cu:220-249: sigma_max = V_max ln(1/R) (m+1)/(2L), (1-i/N)^m).
  *     Pinned by the sigma closed form and a reflection bound vs a large box.
+ *     The reference profile (PMLgamma_func as setPMLcoeffs indexes it along
+ *     x, DTmxw.cu:219-253) is pinned by its flat outer and zero inner
+ *     3-half-cell margins, the round trip of its graded part and the same
+ *     reflection bound, in the CPML and the split-field layer.
  *   - TF/SF corrections, genAlg.py:207-234 with inSF signal.hpp:153-160;
  *     dipole incident field signal.hpp:51-97, plane wave :14-50.  Pinned by
  *     zero leakage outside the box without a scatterer and the closed-form
@@ -291,22 +295,33 @@ static int tfsf_active(const orc_sim* s, long long n) {
 
 /* ------------------------------------------------------- CPML (A4) tables */
 
+/* sigma u half-cells in from the outer face of a layer of p cells.
+ * profile 0: sigma_max = V_max ln(1/R) (m+1)/(2 L) (DTmxw.cu:225), grading
+ * sigma_max (depth/L)^m over the whole layer (DTmxw.cu:226-228).
+ * profile 1: PMLgamma_func(i - 3, p' - 3) of DTmxw.cu:219-228 as
+ * setPMLcoeffs calls it for x (DTmxw.cu:252), with i = u and p' = 2p half-
+ * cells: i - 3 clamped at 0, zero from i - 3 >= N = p' - 6 on, sigma_max =
+ * V ln(1/R) (m+1) / (2 N d/2) and sigma_max (1 - (i-3)/N)^m. */
+static double orc_layer_sigma(int u, int p, double d, double m, double R, double vmax, int profile) {
+  if (profile == DTM_PML_PROFILE_REFERENCE) {
+    double N = 2.0 * p - 6;
+    double a = u - 3 < 0 ? 0 : u - 3;
+    if (a >= N) return 0;
+    double smax = vmax * log(1.0 / R) * (m + 1) / (2 * (N * d * 0.5));
+    double x = pow(smax, 1.0 / m) * (1 - a / N);
+    return pow(x, m);
+  }
+  double L = p * d;
+  double smax = vmax * log(1.0 / R) * (m + 1) / (2 * L);
+  double depth = (double)(2 * p - u) / (2.0 * p);
+  return smax * pow(depth, m);
+}
+
 /* sigma at half-coordinate h along an axis of n cells with thickness
- * plo/phi cells.  sigma_max = V_max ln(1/R) (m+1)/(2 L) (DTmxw.cu:225),
- * grading sigma_max (depth/L)^m (DTmxw.cu:226-228). */
-double orc_pml_sigma(int h, int n, int plo, int phi, double d, double m, double R, double vmax) {
-  if (plo > 0 && h < 2 * plo) {
-    double L = plo * d;
-    double smax = vmax * log(1.0 / R) * (m + 1) / (2 * L);
-    double depth = (double)(2 * plo - h) / (2.0 * plo);
-    return smax * pow(depth, m);
-  }
-  if (phi > 0 && h > 2 * (n - phi)) {
-    double L = phi * d;
-    double smax = vmax * log(1.0 / R) * (m + 1) / (2 * L);
-    double depth = (double)(h - 2 * (n - phi)) / (2.0 * phi);
-    return smax * pow(depth, m);
-  }
+ * plo/phi cells */
+double orc_pml_sigma(int h, int n, int plo, int phi, double d, double m, double R, double vmax, int profile) {
+  if (plo > 0 && h < 2 * plo) return orc_layer_sigma(h, plo, d, m, R, vmax, profile);
+  if (phi > 0 && h > 2 * (n - phi)) return orc_layer_sigma(2 * n - h, phi, d, m, R, vmax, profile);
   return 0;
 }
 
@@ -324,7 +339,7 @@ static void build_cpml(orc_sim* s) {
     s->sk2[a] = (double*)calloc(2 * nn[a] + 1, sizeof(double));
     for (int h = 0; h <= 2 * nn[a]; h++) {
       double sg = orc_pml_sigma(h, nn[a], c->npml[2 * a], c->npml[2 * a + 1], dd[a], c->pml_order,
-                                c->pml_R, c->pml_vmax);
+                                c->pml_R, c->pml_vmax, c->pml_profile);
       /* split-field coefficients, setPMLcoeffs (DTmxw.cu:250-265) */
       s->sk2[a][h] = 1.0 / (1.0 + 0.5 * c->dt * sg);
       s->sk1[a][h] = 2.0 * s->sk2[a][h] - 1.0;

@@ -50,7 +50,7 @@ def lib(fp32=False):
                                       C.c_double]),
         "orc_wave": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
         "orc_pml_sigma": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
-                                       C.c_double, C.c_double]),
+                                       C.c_double, C.c_double, C.c_int]),
         "orc_disp_coeffs": (None, [C.POINTER(Material), C.c_double, P, P, P]),
         "orc_gyro_coeffs": (None, [C.POINTER(Material), C.c_double, P, P]),
         "orc_set_eps": (None, [C.c_double] * 6 + [P]),
@@ -160,8 +160,8 @@ def wave(waveform, w, kEnv, t0, tw, t):
     return lib().orc_wave(waveform, w, kEnv, t0, tw, t)
 
 
-def pml_sigma(h, n, plo, phi, d, m, R, vmax):
-    return lib().orc_pml_sigma(h, n, plo, phi, d, m, R, vmax)
+def pml_sigma(h, n, plo, phi, d, m, R, vmax, profile=0):
+    return lib().orc_pml_sigma(h, n, plo, phi, d, m, R, vmax, profile)
 
 
 def disp_coeffs(mat, dt):

@@ -66,6 +66,8 @@ def legacy_config(cv):
     c = abi.default_config(cv.GridNx, cv.GridNy, cv.GridNz, cv.ds, cv.da, cv.dv, cv.dt)
     c.npml[:] = [cv.Npmlx, cv.Npmlx, cv.Npmly, cv.Npmly, 0, cv.Npmlz]  # no layer at low z (DTmxw.cu:262)
     c.pml_vmax = sp.V_max if sp.V_max > 0 else 1.0
+    c.pml_kind = abi.PML_SPLIT                      # the reference's layer and sigma profile
+    c.pml_profile = abi.PML_PROFILE_REFERENCE
     c.ntime = cv.Ntime
     eps = [1.0, 5.0, 5.0, 4.0, 1e4, 1, 4, 9, 16, 25, 1, 1, 1.75 ** 2, 1.8 ** 2, 1.5 ** 2]
     c.nmat = 15

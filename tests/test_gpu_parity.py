@@ -273,6 +273,23 @@ def test_split_field_layer_plane_wave(inset, sweep):
     check_fields(g, o)
 
 
+@pytest.mark.parametrize("kind", [abi.PML_CPML, abi.PML_SPLIT])
+def test_reference_sigma_profile(kind, sweep):
+    """The reference's sigma grading (DTM_PML_PROFILE_REFERENCE: flat over the
+    outer 3 half-cells, zero over the inner 3) in the CPML and in the
+    split-field layer, plane wave with its TF/SF bands reaching into the layer."""
+    w = configs.plane_wave_sphere(nx=56, ny=40, nz=40, npml=8, inset=0, radius=8, steps=300, periods_w=1.0,
+                                  periods_t0=3.0)
+    w.cfg.pml_kind = kind
+    w.cfg.pml_profile = abi.PML_PROFILE_REFERENCE
+    if sweep == abi.SWEEP_FUSED and kind == abi.PML_SPLIT:
+        with pytest.raises(abi.DTMError):
+            solver(w.cfg)
+        return
+    g, o = run_pair(w)
+    check_fields(g, o)
+
+
 def test_split_field_layer_dipole(sweep):
     w = configs.vacuum_dipole(n=40, npml=8, steps=300)
     w.cfg.pml_kind = abi.PML_SPLIT

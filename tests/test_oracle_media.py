@@ -4,7 +4,9 @@
   continuous profile equals the target R exactly, so the discretely
   integrated sigma must give R (pins sigma_max's factor (m+1)/(2L) and ln R);
 * CPML absorbs: a pulse in a short box agrees with a long box (no
-  reflection) to the discretisation level;
+  reflection) to the discretisation level; so does the reference's sigma
+  grading (DTM_PML_PROFILE_REFERENCE) in the CPML and the split-field layer,
+  whose flat outer and zero inner margins are pinned on the profile itself;
 * the Lorentz/Drude ADE recursion (dispersion.cpp:41-99) has a discrete
   transfer function that converges at second order to the continuous
   chi(w) = (a0 - i w a1)/(b0 - i w b1 - w^2) and eps(w) = eps_inf + chi + i sigma/w;
@@ -34,11 +36,35 @@ def test_pml_sigma_integrates_to_target_reflection():
         np.testing.assert_allclose(sh, s)
 
 
-def pulse_run(nx, steps, npml):
+@pytest.mark.parametrize("p", [4, 8, 15, 22])
+def test_reference_sigma_profile(p):
+    """DTM_PML_PROFILE_REFERENCE is PMLgamma_func (DTmxw.cu:219-228) as
+    setPMLcoeffs indexes it along x (DTmxw.cu:251-253): sigma_max over the
+    outer 3 half-cells (the clamp of i - 3 < 0), zero over the inner 3 (the
+    early return at i >= N - 3), and between them the m-th power grading
+    whose round trip gives exactly R; both sides of an axis alike."""
+    n, d, m, R, V = 100, 0.01, 4.0, 1e-4, 1.0
+    u = np.arange(0, 2 * p + 1)
+    s = np.array([orc.pml_sigma(int(x), n, p, p, d, m, R, V, 1) for x in u])
+    N = 2 * p - 6
+    assert np.all(s[:4] == s[0]) and s[0] == s.max() and s[0] > 0
+    assert np.all(s[2 * p - 3:] == 0.0) and s[2 * p - 4] > 0
+    assert np.all(np.diff(s) <= 0)
+    if N >= 10:  # the graded part's round trip, -2 int sigma / V = ln R (trapezoid on half-cells)
+        integral = np.trapezoid(s[3:3 + N + 1], dx=d / 2)
+        assert 2 * integral / V == pytest.approx(np.log(1 / R), rel=0.03)
+    hi = np.array([orc.pml_sigma(int(2 * n - x), n, p, p, d, m, R, V, 1) for x in u])
+    np.testing.assert_array_equal(hi, s)
+    # the default grading is untouched by the option
+    assert orc.pml_sigma(0, n, p, p, d, m, R, V, 0) == orc.pml_sigma(0, n, p, p, d, m, R, V)
+
+
+def pulse_run(nx, steps, npml, kind=abi.PML_CPML, profile=abi.PML_PROFILE_POLY):
     d = 0.05
     cfg = abi.default_config(nx, 4, 4, d, d, d, configs.dt_for(d, d, d, 0.5))
     cfg.bc[1] = cfg.bc[2] = abi.BC_PERIODIC
     cfg.npml[0] = cfg.npml[1] = npml
+    cfg.pml_kind, cfg.pml_profile = kind, profile
     cfg.nsoft = 1
     so = cfg.soft[0]
     so.comp, so.i, so.j, so.k = abi.EZ, nx // 2, 0, 0
@@ -58,6 +84,18 @@ def test_cpml_absorbs_outgoing_pulse():
     assert peak > 0
     refl = np.abs(short - long_).max() / peak
     assert refl < 5e-3
+
+
+@pytest.mark.parametrize("kind", [abi.PML_CPML, abi.PML_SPLIT])
+def test_reference_profile_absorbs_outgoing_pulse(kind):
+    """The reference's sigma grading absorbs in either layer kind: its 3-cell
+    flat and zero margins cost little against the default grading."""
+    steps = 900
+    long_ = pulse_run(600, steps, 16)
+    short = pulse_run(140, steps, 16, kind, abi.PML_PROFILE_REFERENCE)
+    refl = np.abs(short - long_).max() / np.abs(long_).max()
+    print("reflection", refl)
+    assert refl < 1e-2
 
 
 def ade_transfer(mat, dt, w):
