@@ -24,11 +24,14 @@ struct Grid {
   float dt;
   const void* zero;  // 256 zero bytes
   unsigned zidx;     // = nx*ny*nz: index of the zero pad that ends every field and class array
+  long long zidxw;   // the same index in 64 bits (grids of 2^32 - kPad cells and more)
 };
 
 // Every field array and the class array hold kPad zero elements after the
 // last cell; loads of positions outside the grid read index nx*ny*nz.
 constexpr int kPad = 64;
+// Grids whose cell indices need 64 bits (the sweeps' wide variants).
+inline bool wide_grid(const Grid& g) { return g.zidxw > 0xffffffffLL - kPad; }
 
 // CPML: psi lives only in the slabs.  For axis a the two affected
 // components are (a+1)%3 (slot 0) and (a+2)%3 (slot 1).

@@ -437,12 +437,14 @@ __device__ __forceinline__ T ldr(const T* F, long long base, long long off) {
 // Address of a position under the boundary rule: outside the grid (base or
 // off < 0) it is the array's zero pad, so the load itself is unconditional
 // and its destination register never needs a merge (a merge would force a
-// wait for the load before the next plane).  The index is 32-bit (grids up
-// to 2^32 - kPad cells), which keeps address arithmetic out of 64-bit
-// registers.
-template <typename T>
-__device__ __forceinline__ const T* at(const T* F, long long base, long long off, unsigned zidx) {
-  const unsigned idx = (base < 0 || off < 0) ? zidx : (unsigned)(base + off);
+// wait for the load before the next plane).  The index type I is 32-bit
+// below 2^32 - kPad cells, which keeps the select out of 64-bit registers,
+// and 64-bit in the wide variants (IxT).
+template <bool W> struct IxT { using T = unsigned; };
+template <> struct IxT<true> { using T = unsigned long long; };
+template <typename T, typename I>
+__device__ __forceinline__ const T* at(const T* F, long long base, long long off, I zidx) {
+  const I idx = (base < 0 || off < 0) ? zidx : (I)(base + off);
   return F + idx;
 }
 
@@ -484,9 +486,9 @@ __device__ __forceinline__ void stv2(__half* F, long long q, const float* v, flo
       __floats2half2_rn(__builtin_fmaf(v[0], s, 0.0f), __builtin_fmaf(v[1], s, 0.0f));
 }
 // word of V cells at a position under the boundary rule (see at())
-template <typename P, typename T>
-__device__ __forceinline__ P ldv(const T* F, long long base, long long off, unsigned zidx) {
-  const unsigned idx = (base < 0 || off < 0) ? zidx : (unsigned)(base + off);
+template <typename P, typename T, typename I>
+__device__ __forceinline__ P ldv(const T* F, long long base, long long off, I zidx) {
+  const I idx = (base < 0 || off < 0) ? zidx : (I)(base + off);
   return *reinterpret_cast<const P*>(F + idx);
 }
 
@@ -507,7 +509,8 @@ template <typename P, typename T> struct PreH {
 // genAlg.py:219-221 is added to the derivative it belongs to before the CPML
 // and split-field updates, which are linear in it (the k_tfsf kernel applies
 // the same terms after a sweep); fold: this half-step's factor table.
-template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false>
+// W: 64-bit cell indices (grids of 2^32 - kPad cells and more).
+template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false, bool W = false>
 __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> f, Scales sc, Cpml pm, MarchSet ms,
                                                          const float* fold) {
   using T = typename Raw<STORE>::T;
@@ -540,6 +543,8 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
   T* Hz = f.H[2];
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
+  using Ix = typename IxT<W>::T;
+  const Ix zi = W ? (Ix)g.zidxw : (Ix)g.zidx;  // zero pad index
 
   const long long hq = inreg ? own : -1;  // H operands: owned cells only
   // FP32 zone (FP16 storage): a block whose tile (with its halo) meets the
@@ -565,30 +570,30 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
   auto fetch = [&](PreH<P, T>& S, int p) {
     const long long pb = plane_base<GX>(g, p);
     const long long pn = plane_base<GX>(g, p + 1);
-    S.ex = ldv<P>(Ex, pb, own, g.zidx);
-    S.ey = ldv<P>(Ey, pn, own, g.zidx);
-    S.ez = ldv<P>(Ez, pn, own, g.zidx);
-    S.exy = ldv<P>(Ex, pb, yh, g.zidx);
-    S.ezy = ldv<P>(Ez, pn, yh, g.zidx);
-    S.exz = *at(Ex, pb, zh, g.zidx);
-    S.eyz = *at(Ey, pn, zh, g.zidx);
-    S.hx = ldv<P>((const T*)Hx, pb, hq, g.zidx);
-    S.hy = ldv<P>((const T*)Hy, pb, hq, g.zidx);
-    S.hz = ldv<P>((const T*)Hz, pb, hq, g.zidx);
+    S.ex = ldv<P>(Ex, pb, own, zi);
+    S.ey = ldv<P>(Ey, pn, own, zi);
+    S.ez = ldv<P>(Ez, pn, own, zi);
+    S.exy = ldv<P>(Ex, pb, yh, zi);
+    S.ezy = ldv<P>(Ez, pn, yh, zi);
+    S.exz = *at(Ex, pb, zh, zi);
+    S.eyz = *at(Ey, pn, zh, zi);
+    S.hx = ldv<P>((const T*)Hx, pb, hq, zi);
+    S.hy = ldv<P>((const T*)Hy, pb, hq, zi);
+    S.hz = ldv<P>((const T*)Hz, pb, hq, zi);
   };
   // Ey / Ez tile of plane p straight into its ring slot (prologue only)
   auto ring_direct = [&](int p) {
     const long long pb = plane_base<GX>(g, p);
     const int s = ring5(p);
     float v[V];
-    unpackv(ldv<P>(Ey, pb, own, g.zidx), iey, v);
+    unpackv(ldv<P>(Ey, pb, own, zi), iey, v);
 #pragma unroll
     for (int q = 0; q < V; q++) rEy[s][ty][c0 + q + 2] = v[q];
-    unpackv(ldv<P>(Ez, pb, own, g.zidx), iez, v);
+    unpackv(ldv<P>(Ez, pb, own, zi), iez, v);
 #pragma unroll
     for (int q = 0; q < V; q++) rEz[s][ty + 2][c0 + q] = v[q];
     if (yw) {
-      unpackv(ldv<P>(Ez, pb, yh, g.zidx), iez, v);
+      unpackv(ldv<P>(Ez, pb, yh, zi), iez, v);
 #pragma unroll
       for (int q = 0; q < V; q++) rEz[s][yrow][c0 + q] = v[q];
     }
@@ -1049,7 +1054,8 @@ template <> struct ClsWord<2> { using T = unsigned short; };
 
 // MAT: the grid may hold dispersive / gyrotropic cells or a design box; the
 // variants without (MAT false) carry none of their branches and pointers.
-template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false, bool MAT = true>
+template <int STORE, int GEN, int V, bool SP = false, int ROWS = 8, bool XZ = false, bool TF = false, bool MAT = true,
+          bool W = false>
 __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt,
                                                          Ade ade, MarchSet ms, const float* fold) {
   using T = typename Raw<STORE>::T;
@@ -1087,6 +1093,8 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
   T* Ez = f.E[2];
   const float ihx = sc.ih[0], ihy = sc.ih[1], ihz = sc.ih[2];
   const float iex = sc.ie[0], iey = sc.ie[1], iez = sc.ie[2];
+  using Ix = typename IxT<W>::T;
+  const Ix zi = W ? (Ix)g.zidxw : (Ix)g.zidx;  // zero pad index
 
   const long long eq = inreg ? own : -1;  // E operands: owned cells only
   // FP32 zone (FP16 storage), as in the H sweep
@@ -1110,34 +1118,34 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
   // with no class array every cell is class 0: read the zero page
   const uint8_t* clsp = mt.cls ? mt.cls : static_cast<const uint8_t*>(g.zero);
   const long long cq = mt.cls ? eq : 0;
-  const unsigned czidx = mt.cls ? g.zidx : 0u;
+  const Ix czidx = mt.cls ? zi : (Ix)0;
   auto fetch = [&](PreE<P, T, CL>& S, int p) {
     const long long pb = plane_base<GX>(g, p);
     const long long pn = plane_base<GX>(g, p + 2);
-    S.hx = ldv<P>(Hx, pb, own, g.zidx);
-    S.hy = ldv<P>(Hy, pn, own, g.zidx);
-    S.hz = ldv<P>(Hz, pn, own, g.zidx);
-    S.hxy = ldv<P>(Hx, pb, yh, g.zidx);
-    S.hzy = ldv<P>(Hz, pn, yh, g.zidx);
-    S.hxz = *at(Hx, pb, zh, g.zidx);
-    S.hyz = *at(Hy, pn, zh, g.zidx);
-    S.ex = ldv<P>((const T*)Ex, pb, eq, g.zidx);
-    S.ey = ldv<P>((const T*)Ey, pb, eq, g.zidx);
-    S.ez = ldv<P>((const T*)Ez, pb, eq, g.zidx);
+    S.hx = ldv<P>(Hx, pb, own, zi);
+    S.hy = ldv<P>(Hy, pn, own, zi);
+    S.hz = ldv<P>(Hz, pn, own, zi);
+    S.hxy = ldv<P>(Hx, pb, yh, zi);
+    S.hzy = ldv<P>(Hz, pn, yh, zi);
+    S.hxz = *at(Hx, pb, zh, zi);
+    S.hyz = *at(Hy, pn, zh, zi);
+    S.ex = ldv<P>((const T*)Ex, pb, eq, zi);
+    S.ey = ldv<P>((const T*)Ey, pb, eq, zi);
+    S.ez = ldv<P>((const T*)Ez, pb, eq, zi);
     S.cls = ldv<CL>(clsp, mt.cls ? pb : 0, cq, czidx);
   };
   auto ring_direct = [&](int p) {  // Hy / Hz tile of plane p into its slot
     const long long pb = plane_base<GX>(g, p);
     const int s = ring5(p);
     float v[V];
-    unpackv(ldv<P>(Hy, pb, own, g.zidx), ihy, v);
+    unpackv(ldv<P>(Hy, pb, own, zi), ihy, v);
 #pragma unroll
     for (int q = 0; q < V; q++) rHy[s][ty][c0 + q + 1] = v[q];
-    unpackv(ldv<P>(Hz, pb, own, g.zidx), ihz, v);
+    unpackv(ldv<P>(Hz, pb, own, zi), ihz, v);
 #pragma unroll
     for (int q = 0; q < V; q++) rHz[s][ty + 1][c0 + q] = v[q];
     if (yw) {
-      unpackv(ldv<P>(Hz, pb, yh, g.zidx), ihz, v);
+      unpackv(ldv<P>(Hz, pb, yh, zi), ihz, v);
 #pragma unroll
       for (int q = 0; q < V; q++) rHz[s][yrow][c0 + q] = v[q];
     }
@@ -1563,56 +1571,56 @@ static bool use_16_rows(int slots8, int slots16, const Region* regs, int nreg, i
   return c16 <= c8;
 }
 
-template <int STORE, int V, bool SP, bool XZ, bool TF>
+template <int STORE, int V, bool SP, bool XZ, bool TF, bool W = false>
 static void upd_h(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Region* regs, int nreg,
                   const float* fold, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   int seg8 = 0, seg16 = 0;
   if constexpr (V == 1) {
-    static const int slots8 = resident_blocks(k_update_h<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
-    static const int slots16 = resident_blocks(k_update_h<STORE, 3, V, SP, 16, XZ, TF>, BZ * 16);
+    static const int slots8 = resident_blocks(k_update_h<STORE, 3, V, SP, 8, XZ, TF, W>, BZ * 8);
+    static const int slots16 = resident_blocks(k_update_h<STORE, 3, V, SP, 16, XZ, TF, W>, BZ * 16);
     if (use_16_rows(slots8, slots16, regs, nreg, BZ, seg8, seg16)) {
       MarchSet rg = make_march(regs, nreg, seg16, BZ, 16);
       if (rg.total <= 0) return;
-      hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 16, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f, sc, pm,
-                         rg, fold);
+      hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 16, XZ, TF, W>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g, f,
+                         sc, pm, rg, fold);
       check_launch("k_update_h");
       return;
     }
   } else {
-    static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP, 8, XZ, TF>, BZ * 8);
+    static const int slots = resident_blocks(k_update_h<STORE, 3, V, SP, 8, XZ, TF, W>, BZ * 8);
     seg8 = march_seg(regs, nreg, slots, BZ * V, 8);
   }
   MarchSet rg = make_march(regs, nreg, seg8, BZ * V, 8);
   if (rg.total <= 0) return;
-  hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 8, XZ, TF>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm, rg,
-                     fold);
+  hipLaunchKernelGGL((k_update_h<STORE, 3, V, SP, 8, XZ, TF, W>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc, pm,
+                     rg, fold);
   check_launch("k_update_h");
 }
 
-template <int STORE, int V, bool SP, bool XZ, bool TF, bool MAT = true>
+template <int STORE, int V, bool SP, bool XZ, bool TF, bool MAT = true, bool W = false>
 static void upd_e(const Grid& g, const void* fields, const Scales& sc, const Cpml& pm, const Materials& mt,
                   const Ade& ade, const Region* regs, int nreg, const float* fold, hipStream_t st) {
   const Fields<STORE>& f = *static_cast<const Fields<STORE>*>(fields);
   int seg8 = 0, seg16 = 0;
   if constexpr (V == 1) {
-    static const int slots8 = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT>, BZ * 8);
-    static const int slots16 = resident_blocks(k_update_e<STORE, 3, V, SP, 16, XZ, TF, MAT>, BZ * 16);
+    static const int slots8 = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT, W>, BZ * 8);
+    static const int slots16 = resident_blocks(k_update_e<STORE, 3, V, SP, 16, XZ, TF, MAT, W>, BZ * 16);
     if (use_16_rows(slots8, slots16, regs, nreg, BZ, seg8, seg16)) {
       MarchSet rg = make_march(regs, nreg, seg16, BZ, 16);
       if (rg.total <= 0) return;
-      hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 16, XZ, TF, MAT>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g,
+      hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 16, XZ, TF, MAT, W>), dim3(rg.per * NXCD), dim3(BZ, 16, 1), 0, st, g,
                          f, sc, pm, mt, ade, rg, fold);
       check_launch("k_update_e");
       return;
     }
   } else {
-    static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT>, BZ * 8);
+    static const int slots = resident_blocks(k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT, W>, BZ * 8);
     seg8 = march_seg(regs, nreg, slots, BZ * V, 8);
   }
   MarchSet rg = make_march(regs, nreg, seg8, BZ * V, 8);
   if (rg.total <= 0) return;
-  hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc,
+  hipLaunchKernelGGL((k_update_e<STORE, 3, V, SP, 8, XZ, TF, MAT, W>), dim3(rg.per * NXCD), dim3(BZ, 8, 1), 0, st, g, f, sc,
                      pm, mt, ade, rg, fold);
   check_launch("k_update_e");
 }
@@ -1637,6 +1645,12 @@ static void update_h_go(int store, const Grid& g, const void* fields, const Scal
                         const Region* regs, int nreg, const float* fold, hipStream_t st) {
   const bool xz = store == 1 && sc.xz.on;
   const bool pairs = pair_cells(store, g, nullptr) && even_k(regs, nreg);
+  if (wide_grid(g)) {  // 64-bit cell indices (no split layer or zone there: dtm_create refuses them)
+    if (store == 0) upd_h<0, 1, false, false, TF, true>(g, fields, sc, pm, regs, nreg, fold, st);
+    else if (pairs) upd_h<1, 2, false, false, TF, true>(g, fields, sc, pm, regs, nreg, fold, st);
+    else upd_h<1, 1, false, false, TF, true>(g, fields, sc, pm, regs, nreg, fold, st);
+    return;
+  }
   if (pm.split) {  // split-field layer: one cell per lane
     if (store == 0) upd_h<0, 1, true, false, TF>(g, fields, sc, pm, regs, nreg, fold, st);
     else if (xz) upd_h<1, 1, true, true, false>(g, fields, sc, pm, regs, nreg, fold, st);
@@ -1660,6 +1674,12 @@ static void update_e_go(int store, const Grid& g, const void* fields, const Scal
   const bool xz = store == 1 && sc.xz.on;
   const bool pairs = pair_cells(store, g, &mt) && even_k(regs, nreg);
   const bool mat = ade.on || mt.d_on;  // else the variants without the dispersive / design branches
+  if (wide_grid(g)) {  // 64-bit cell indices, the material branches kept
+    if (store == 0) upd_e<0, 1, false, false, TF, true, true>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else if (pairs) upd_e<1, 2, false, false, TF, true, true>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    else upd_e<1, 1, false, false, TF, true, true>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
+    return;
+  }
   if (pm.split) {
     if (store == 0) upd_e<0, 1, true, false, TF>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);
     else if (xz) upd_e<1, 1, true, true, false>(g, fields, sc, pm, mt, ade, regs, nreg, fold, st);

@@ -558,6 +558,7 @@ static int setup_cpml(dtm_solver* s) {
     size_t cnt = 0;
     if (P > 0 && !p.split)
       cnt = (size_t)P * (a == 0 ? (size_t)c.ny * c.nz : a == 1 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny);
+    if (cnt >= (1ull << 32)) return fail(DTM_ERR_LIMIT, "a CPML slab holds 2^32 cells or more (32-bit psi indices)");
     for (int sl = 0; sl < 2; sl++) {
       if (int rc = sdalloc(s, &p.psiE[a][sl], sizeof(float) * cnt)) return rc;
       if (int rc = sdalloc(s, &p.psiH[a][sl], sizeof(float) * cnt)) return rc;
@@ -631,6 +632,8 @@ static void swap_buffers(dtm_solver* s) {
 }
 
 static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]);
+// class of cell q (an empty class array: every cell is class 0)
+static int cls_of(const dtm_solver* s, size_t q) { return s->cls_host.empty() ? 0 : s->cls_host[q]; }
 
 // Build the class table from per-component material ids.  A volume that is
 // rejected leaves the solver's current materials in place.
@@ -653,7 +656,7 @@ static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]) {
   std::unordered_map<uint32_t, int> ids;
   s->classes.clear();
   s->class_mats.clear();
-  s->cls_host.assign(n, 0);
+  s->cls_host.clear();
   auto entry_for = [&](uint8_t a, uint8_t b, uint8_t d) -> int {
     const uint32_t key = a | (b << 8) | (d << 16);
     auto it = ids.find(key);
@@ -677,7 +680,11 @@ static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]) {
   entry_for(0, 0, 0);
   int bb[6] = {c.nx, -1, c.ny, -1, c.nz, -1};
   bool any_special = false, any_gyro = false;
-  for (size_t q = 0; q < n; q++) {
+  // no volumes and a plain material 0: every cell is class 0, and no
+  // per-cell class array exists (cls_of)
+  const bool uniform = !m[0] && !m[1] && !m[2] && !s->classes[0].special;
+  if (!uniform) s->cls_host.assign(n, 0);
+  for (size_t q = 0; q < (uniform ? 0 : n); q++) {
     const uint8_t a = m[0] ? m[0][q] : 0, b = m[1] ? m[1][q] : 0, d = m[2] ? m[2][q] : 0;
     if (a >= c.nmat || b >= c.nmat || d >= c.nmat) return fail(DTM_ERR_ARG, "material id >= nmat");
     const int id = entry_for(a, b, d);
@@ -717,14 +724,14 @@ static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]) {
     for (int i = mt.d_i0; i < mt.d_i0 + mt.d_ni; i++)
       for (int j = mt.d_j0; j < mt.d_j0 + mt.d_nj; j++)
         for (int k = mt.d_k0; k < mt.d_k0 + mt.d_nk; k++)
-          if (s->classes[s->cls_host[((size_t)i * c.ny + j) * c.nz + k]].special)
+          if (s->classes[cls_of(s, ((size_t)i * c.ny + j) * c.nz + k)].special)
             return fail(DTM_ERR_ARG, "the design box overlaps a dispersive or gyrotropic cell");
   }
   // soft sources must sit in plain cells
   for (int q = 0; q < c.nsoft; q++) {
     const dtm_soft_source& so = c.soft[q];
     const size_t idx = ((size_t)so.i * c.ny + so.j) * c.nz + so.k;
-    if (s->classes[s->cls_host[idx]].special)
+    if (s->classes[cls_of(s, idx)].special)
       return fail(DTM_ERR_ARG, "soft source placed in a dispersive/gyrotropic cell");
   }
   const int ncls = (int)s->classes.size();
@@ -788,8 +795,12 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
   *out = nullptr;
   const dtm_config& c = *cfg;
   if (c.nx < 1 || c.ny < 1 || c.nz < 1) return fail(DTM_ERR_ARG, "grid sizes must be positive");
-  if ((unsigned long long)c.nx * c.ny * c.nz > 0xffffffffULL - kPad)
-    return fail(DTM_ERR_LIMIT, "grids are limited to 2^32 - 64 cells per solver");
+  // 2^32 cells and more: the sweeps index cells with 64 bits (their wide
+  // variants), which the split-field layer, the FP32 zone and the fused step
+  // do not have
+  const bool wide = (unsigned long long)c.nx * c.ny * c.nz > 0xffffffffULL - kPad;
+  if (wide && c.pml_kind == DTM_PML_SPLIT)
+    return fail(DTM_ERR_LIMIT, "the split-field layer is limited to grids below 2^32 cells");
   if ((long long)c.nx * c.ny * c.nz > (1LL << 34)) return fail(DTM_ERR_LIMIT, "grid too large");
   if (!(c.dx > 0 && c.dy > 0 && c.dz > 0 && c.dt > 0)) return fail(DTM_ERR_ARG, "steps must be positive");
   const double cour = c.dt * sqrt(1 / (c.dx * c.dx) + 1 / (c.dy * c.dy) + 1 / (c.dz * c.dz));
@@ -879,7 +890,8 @@ int dtm_create(const dtm_config* cfg, dtm_solver** out) {
     }
     g.zero = z;
   }
-  g.zidx = (unsigned)((size_t)c.nx * c.ny * c.nz);
+  g.zidx = (unsigned)((size_t)c.nx * c.ny * c.nz);  // wide grids use zidxw
+  g.zidxw = (long long)((size_t)c.nx * c.ny * c.nz);
   const size_t n = (size_t)c.nx * c.ny * c.nz;
   const size_t esz = s->store ? 2 : 4;
   // the six components of a field set share one allocation (Ex Ey Ez Hx Hy
@@ -951,7 +963,7 @@ int dtm_set_design_eps(dtm_solver* s, int i0, int j0, int k0, int ni, int nj, in
     for (int j = 0; j < nj; j++)
       for (int k = 0; k < nk; k++) {
         const size_t q = ((size_t)(i0 + i) * c.ny + (j0 + j)) * c.nz + (k0 + k);
-        const ClassEntry& ce = s->classes[s->cls_host[q]];
+        const ClassEntry& ce = s->classes[cls_of(s, q)];
         if (ce.special) return fail(DTM_ERR_ARG, "design box overlaps a dispersive cell");
       }
   s->design_host.resize(nb);
@@ -1162,7 +1174,7 @@ static int one_step(dtm_solver* s) {
     for (int q = 0; q < c.nsoft; q++) {
       const dtm_soft_source& ss = c.soft[q];
       const size_t idx = ((size_t)ss.i * c.ny + ss.j) * c.nz + ss.k;
-      const ClassEntry& ce = s->classes[s->cls_host[idx]];
+      const ClassEntry& ce = s->classes[cls_of(s, idx)];
       double ie = ce.inv_eps[ss.comp];
       if (s->mt.d_on) {
         const int di = ss.i - s->mt.d_i0, dj = ss.j - s->mt.d_j0, dk = ss.k - s->mt.d_k0;
@@ -1352,7 +1364,7 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     if (in_layer(i, j, k))
       return fail(DTM_ERR_ARG, "a monitor lies in the CPML (the adjoint is exact outside the layer only)");
     q[m] = ((long long)i * c.ny + j) * c.nz + k;
-    ie[m] = s->classes[s->cls_host[q[m]]].inv_eps[2];
+    ie[m] = s->classes[cls_of(s, (size_t)q[m])].inv_eps[2];
     const int di = i - mt.d_i0, dj = j - mt.d_j0, dk = k - mt.d_k0;
     if (di >= 0 && di < mt.d_ni && dj >= 0 && dj < mt.d_nj && dk >= 0 && dk < mt.d_nk) {
       ie[m] = s->design_host[((size_t)di * mt.d_nj + dj) * mt.d_nk + dk];
@@ -1808,6 +1820,7 @@ int dtm_set_fp32_zone(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int
   if (s->store != DTM_STORE_FP16) return fail(DTM_ERR_ARG, "the FP32 zone applies to FP16 storage only");
   if (s->torre) return fail(DTM_ERR_ARG, "the FP32 zone runs with the two-pass sweeps only");
   if (s->sc.xz.on) return fail(DTM_ERR_STATE, "the FP32 zone is already set");
+  if (wide_grid(s->g)) return fail(DTM_ERR_LIMIT, "the FP32 zone is limited to grids below 2^32 cells");
   if (!box_ok(s, i0, i1, j0, j1, k0, k1)) return fail(DTM_ERR_ARG, "FP32 zone outside the grid or empty");
   const long long cells = (long long)(i1 - i0) * (j1 - j0) * (k1 - k0);
   if (cells > kMaxZoneCells) return fail(DTM_ERR_LIMIT, "FP32 zone larger than 2^27 cells");
