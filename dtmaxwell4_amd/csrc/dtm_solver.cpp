@@ -1375,13 +1375,19 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
   // Checkpoint mode (interval K) keeps the whole state every K steps and,
   // segment by segment on the way back, recomputes the segment forward to
   // rebuild its K+1 snapshots; the adjoint state waits in a stash meanwhile.
-  // The last segment's snapshots are taken during the forward pass, so the
-  // way back starts without a recompute.
+  // The last L segments keep their snapshots from the forward pass instead
+  // of a checkpoint: L = 1 + (checkpoint bytes / segment snapshot bytes), so
+  // the tail never takes more memory than the checkpoints it replaces plus
+  // one segment's snapshots, and the way back starts without a recompute.
   const bool ckpt = s->adj_every > 0 && s->adj_every < nsteps;
   const int K = ckpt ? s->adj_every : nsteps;
-  const int nck = ckpt ? (nsteps + K - 1) / K : 0;
+  const int nseg = ckpt ? (nsteps + K - 1) / K : 1;
   const size_t sbytes = ckpt ? state_bytes(s) : 0;
-  const size_t hist_bytes = sizeof(float) * 3 * (size_t)nd * ((size_t)K + 1);
+  const size_t seg_hist = sizeof(float) * 3 * (size_t)nd * (size_t)K;
+  const int ntail = ckpt ? (int)std::min<size_t>((size_t)nseg, 1 + sbytes / std::max<size_t>(seg_hist, 1)) : 1;
+  const int nck = nseg - ntail;  // segments rebuilt from a checkpoint
+  const int n_last = nck * K;    // first step whose snapshots the forward pass keeps
+  const size_t hist_bytes = sizeof(float) * 3 * (size_t)nd * ((size_t)std::max(K, nsteps - n_last) + 1);
   void *d_q = nullptr, *d_ie = nullptr, *d_ser = nullptr, *d_hist = nullptr, *d_acc = nullptr, *d_F = nullptr;
   void *d_ck = nullptr, *d_stash = nullptr;
   auto release = [&]() {
@@ -1392,7 +1398,7 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
       hipMalloc(&d_ser, sizeof(float) * (size_t)nmon * nsteps) != hipSuccess ||
       hipMalloc(&d_hist, hist_bytes) != hipSuccess || hipMalloc(&d_acc, sizeof(double) * nd) != hipSuccess ||
       hipMalloc(&d_F, sizeof(double) * 2 * nmon) != hipSuccess ||
-      (ckpt && (hipMalloc(&d_ck, sbytes * nck) != hipSuccess || hipMalloc(&d_stash, sbytes) != hipSuccess))) {
+      (nck > 0 && (hipMalloc(&d_ck, sbytes * nck) != hipSuccess || hipMalloc(&d_stash, sbytes) != hipSuccess))) {
     (void)hipGetLastError();
     release();
     return fail(DTM_ERR_NOMEM, "adjoint temporaries (design-box history " + std::to_string(hist_bytes >> 20) +
@@ -1418,10 +1424,9 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     }
     // forward: record Ez at the monitors every step, and the design-box E
     // every step (history) or the whole state every K steps (checkpoints)
-    // plus the last segment's snapshots
-    const int n_last = ckpt ? (nck - 1) * K : 0;  // first step of the last segment
+    // plus the tail segments' snapshots
     for (int n = 0; n < nsteps && !rc; n++) {
-      if (ckpt && n % K == 0) {
+      if (n < n_last && n % K == 0) {
         ck_ptrs[n / K] = state_ptrs(s);
         if ((rc = save_state(s, ck + (size_t)(n / K) * sbytes))) break;
       }
@@ -1460,9 +1465,12 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
     if ((rc = zero_state(s))) break;
     s->adjoint = true;
     launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, nsteps, s->stream);
-    for (int seg = (ckpt ? nck - 1 : 0); seg >= 0 && !rc; seg--) {
+    for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
       const int n0 = seg * K, n1 = std::min(n0 + K, nsteps);
-      if (ckpt && seg < nck - 1) {  // rebuild the segment's history from its checkpoint, the adjoint state stashed
+      // this segment's snapshots: kept from the forward pass (tail) or
+      // rebuilt at the start of the buffer (the tail is done with by then)
+      float* const hs0 = hist + (seg >= nck ? (size_t)(n0 - n_last) * snap : 0);
+      if (seg < nck) {  // rebuild the segment's history from its checkpoint, the adjoint state stashed
         const StateImage adj = state_ptrs(s);
         const long long adj_step = s->step;
         if ((rc = save_state(s, stash)) || (rc = load_state(s, ck + (size_t)seg * sbytes, ck_ptrs[seg]))) break;
@@ -1483,19 +1491,19 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
       // gradient of step n, one backward step, injection at step n; the
       // injection and the next step's gradient share a launch when they
       // touch disjoint cells
-      launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n1 - 1 - n0) * snap, hist + (size_t)(n1 - n0) * snap,
+      launch_adj_grad(s->g, fields_ptr(s), mt, hs0 + (size_t)(n1 - 1 - n0) * snap, hs0 + (size_t)(n1 - n0) * snap,
                       acc, s->stream);
       for (int n = n1 - 1; n >= n0 && !rc; n--) {
         rc = one_step(s);
         const bool next = n - 1 >= n0;  // the gradient of step n-1 follows in this segment
         if (n >= 1 && next && !mon_in_design) {
-          launch_adj_inject_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n - 1 - n0) * snap,
-                                 hist + (size_t)(n - n0) * snap, acc, hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
+          launch_adj_inject_grad(s->g, fields_ptr(s), mt, hs0 + (size_t)(n - 1 - n0) * snap,
+                                 hs0 + (size_t)(n - n0) * snap, acc, hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
           continue;
         }
         if (n >= 1) launch_adj_inject(fields_ptr(s), hq, hie, Fd, nmon, omega, c.dt, n, s->stream);
         if (next)
-          launch_adj_grad(s->g, fields_ptr(s), mt, hist + (size_t)(n - 1 - n0) * snap, hist + (size_t)(n - n0) * snap,
+          launch_adj_grad(s->g, fields_ptr(s), mt, hs0 + (size_t)(n - 1 - n0) * snap, hs0 + (size_t)(n - n0) * snap,
                           acc, s->stream);
       }
     }

@@ -341,10 +341,11 @@ int dtm_adjoint_design_gradient(dtm_solver* s, int nsteps, double omega, int nmo
 /* Memory scheme of dtm_adjoint_design_gradient.  every = 0 (default) keeps
  * the design-box E of every step.  every = K > 0 keeps the whole time-stepped
  * state every K steps instead and, segment by segment on the way back,
- * recomputes each K steps forward to rebuild their design-box E (the last
- * segment's are taken during the forward pass): device memory
+ * recomputes each K steps forward to rebuild their design-box E.  The last
+ * L = 1 + (state bytes / K steps of snapshots) segments keep their snapshots
+ * from the forward pass instead of a checkpoint: device memory at most
  * (ceil(nsteps/K) + 1) states plus K+1 design-box snapshots, for one more
- * forward pass less one segment.  For grids whose design box is a large part
+ * forward pass less L segments.  For grids whose design box is a large part
  * of the domain.
  * No counterpart in the reference (it has no adjoint). */
 int dtm_adjoint_set_checkpoints(dtm_solver* s, int every);
