@@ -257,11 +257,19 @@ int take_launch_error(char* msg, int cap);
 // fold non-null: add a plane-wave TF/SF source along x inside the sweep,
 // from this half-step's per-plane factor table (launch_tfsf_fold); ignored
 // with an FP32 zone
+// side: with an FP32 zone, the zone's column runs on side.st between the
+// fork and join events (concurrently with the rest of the grid on st); null:
+// both launches on st.
+struct SideStream {
+  hipStream_t st;
+  hipEvent_t fork, join;
+};
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Region* regs, int nreg, hipStream_t st, const float* fold = nullptr);
+                     const Region* regs, int nreg, hipStream_t st, const float* fold = nullptr,
+                     const SideStream* side = nullptr);
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                      const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st,
-                     const float* fold = nullptr);
+                     const float* fold = nullptr, const SideStream* side = nullptr);
 // floats per half-step of the fold factor table: (nx + 1) * 8
 void launch_tfsf_fold(const Tfsf& tf, const double* tables, float* out, int nsteps, const Grid& g,
                       hipStream_t st);

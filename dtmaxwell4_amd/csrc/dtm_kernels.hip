@@ -1700,22 +1700,30 @@ static void update_e_go(int store, const Grid& g, const void* fields, const Scal
 }
 
 // With an FP32 zone only a column of cells (the zone's y and z range plus
-// the 3 cells its stencils reach, every x plane) runs the zone variants; the
-// rest of the grid runs the plain ones as up to four boxes around it (y
-// slabs, then z slabs inside the column's y range), all of full x extent, so
-// every launch keeps the long x segments of the whole grid.  Both launches
-// of a sweep write disjoint cells and read only the other family of fields,
-// so their order does not matter.  A periodic axis keeps the column at its
-// full extent on that axis.
+// the 3 cells its stencils reach, widened to the tile grid, every x plane)
+// runs the zone variants; the rest of the grid runs the plain ones as up to
+// four boxes around it (y slabs, then z slabs inside the column's y range),
+// all of full x extent, so every launch keeps the long x segments of the
+// whole grid.  Both launches of a sweep write disjoint cells and read only
+// the other family of fields, so their order does not matter and they may
+// run concurrently.  A periodic axis keeps the column at its full extent on
+// that axis.
 static bool zone_split(const Grid& g, const Exact& x, const Region* regs, int nreg, Region* outer, int& nouter,
                        Region& mid) {
   if (nreg != 1) return false;
   const Region& R = regs[0];
-  const int j0 = g.per[1] ? R.j0 : max(R.j0, x.j0 - 3), j1 = g.per[1] ? R.j1 : min(R.j1, x.j1 + 3);
+  int j0 = g.per[1] ? R.j0 : max(R.j0, x.j0 - 3), j1 = g.per[1] ? R.j1 : min(R.j1, x.j1 + 3);
   int k0 = g.per[2] ? R.k0 : max(R.k0, x.k0 - 3), k1 = g.per[2] ? R.k1 : min(R.k1, x.k1 + 3);
   if (j0 >= j1 || k0 >= k1) return false;
-  // z bounds even: the paired FP16 sweeps load two z cells per lane from an
-  // even k (an odd start would misalign every __half2 word of a box)
+  // the column on the region's tile grid (rows in 16s, z in 128s from its
+  // origin: every tile shape of the plain launch), so the outer boxes hold
+  // whole tiles of the one-box tiling and add no partial tiles; z bounds are
+  // then even, as the paired FP16 sweeps need (two z cells per lane from an
+  // even k)
+  j0 = R.j0 + (j0 - R.j0) / 16 * 16;
+  j1 = min(R.j1, R.j0 + (j1 - R.j0 + 15) / 16 * 16);
+  k0 = R.k0 + (k0 - R.k0) / 128 * 128;
+  k1 = min(R.k1, R.k0 + (k1 - R.k0 + 127) / 128 * 128);
   k0 &= ~1;
   k1 = min(R.k1, (k1 + 1) & ~1);
   nouter = 0;
@@ -1730,15 +1738,32 @@ static bool zone_split(const Grid& g, const Exact& x, const Region* regs, int nr
   return true;
 }
 
+// The zone's column and the outer boxes write disjoint cells and read only
+// the other family of fields: with a side stream they run concurrently.
+static void side_fork(const SideStream* side, hipStream_t st) {
+  if (!side) return;
+  hipError_t e = hipEventRecord(side->fork, st);
+  if (e == hipSuccess) e = hipStreamWaitEvent(side->st, side->fork, 0);
+  if (e != hipSuccess) fail_launch("side stream fork", hipGetErrorString(e));
+}
+static void side_join(const SideStream* side, hipStream_t st) {
+  if (!side) return;
+  hipError_t e = hipEventRecord(side->join, side->st);
+  if (e == hipSuccess) e = hipStreamWaitEvent(st, side->join, 0);
+  if (e != hipSuccess) fail_launch("side stream join", hipGetErrorString(e));
+}
+
 void launch_update_h(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
-                     const Region* regs, int nreg, hipStream_t st, const float* fold) {
+                     const Region* regs, int nreg, hipStream_t st, const float* fold, const SideStream* side) {
   Region outer[4], mid;
   int nouter = 0;
   if (store == 1 && sc.xz.on && zone_split(g, sc.xz, regs, nreg, outer, nouter, mid)) {
     Scales plain = sc;
     plain.xz.on = 0;
+    side_fork(side, st);
+    update_h_go<false>(store, g, fields, sc, pm, &mid, 1, nullptr, side ? side->st : st);
     if (nouter) update_h_go<false>(store, g, fields, plain, pm, outer, nouter, nullptr, st);
-    update_h_go<false>(store, g, fields, sc, pm, &mid, 1, nullptr, st);
+    side_join(side, st);
     return;
   }
   if (fold && !(store == 1 && sc.xz.on))
@@ -1749,14 +1774,16 @@ void launch_update_h(int store, const Grid& g, const void* fields, const Scales&
 
 void launch_update_e(int store, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                      const Materials& mt, const Ade& ade, const Region* regs, int nreg, hipStream_t st,
-                     const float* fold) {
+                     const float* fold, const SideStream* side) {
   Region outer[4], mid;
   int nouter = 0;
   if (store == 1 && sc.xz.on && zone_split(g, sc.xz, regs, nreg, outer, nouter, mid)) {
     Scales plain = sc;
     plain.xz.on = 0;
+    side_fork(side, st);
+    update_e_go<false>(store, g, fields, sc, pm, mt, ade, &mid, 1, nullptr, side ? side->st : st);
     if (nouter) update_e_go<false>(store, g, fields, plain, pm, mt, ade, outer, nouter, nullptr, st);
-    update_e_go<false>(store, g, fields, sc, pm, mt, ade, &mid, 1, nullptr, st);
+    side_join(side, st);
     return;
   }
   if (fold && !(store == 1 && sc.xz.on))

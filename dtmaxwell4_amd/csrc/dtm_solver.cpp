@@ -51,6 +51,7 @@ struct dtm_solver {
   dtm_config cfg;
   int dev = 0;
   hipStream_t stream = nullptr;
+  SideStream side{};  // the FP32 zone's column runs on side.st (set with the zone)
   Grid g{};
   Scales sc{};
   int store = 0;
@@ -939,7 +940,11 @@ void dtm_destroy(dtm_solver* s) {
     (void)hipEventDestroy(e[0]);
     (void)hipEventDestroy(e[1]);
   }
+  if (s->side.st) (void)hipStreamSynchronize(s->side.st);
   for (void* p : s->allocs) (void)hipFree(p);
+  if (s->side.fork) (void)hipEventDestroy(s->side.fork);
+  if (s->side.join) (void)hipEventDestroy(s->side.join);
+  if (s->side.st) (void)hipStreamDestroy(s->side.st);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -1148,7 +1153,9 @@ static int one_step(dtm_solver* s) {
     Tfsf tfh{};
     if (s->tf.active) tfh = tfsf_for(s, n, 0);
     ev_begin(s, 0);
-    launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, s->stream, fold ? fold_for(s, n, 0) : nullptr);
+    const SideStream* side = s->sc.xz.on && s->side.st ? &s->side : nullptr;
+    launch_update_h(s->store, s->g, F, s->sc, s->pm, &s->whole, 1, s->stream, fold ? fold_for(s, n, 0) : nullptr,
+                    side);
     ev_end(s);
     if (s->tf.active && !fold) {
       ev_begin(s, 4);
@@ -1160,7 +1167,7 @@ static int one_step(dtm_solver* s) {
     if (s->tf.active) tfe = tfsf_for(s, n, 1);
     ev_begin(s, 2);
     launch_update_e(s->store, s->g, F, s->sc, s->pm, s->mt, s->ade, &s->whole, 1, s->stream,
-                    fold ? fold_for(s, n, 1) : nullptr);
+                    fold ? fold_for(s, n, 1) : nullptr, side);
     ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, F, s->sc, s->mt, s->ade, s->stream);
@@ -1835,6 +1842,11 @@ int dtm_set_fp32_zone(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int
   Exact x{};
   for (int c = 0; c < 6; c++)
     if (int rc = sdalloc(s, &x.f[c], sizeof(float) * (size_t)cells)) return rc;
+  if (!s->side.st) {  // a second stream (default priority) for the zone's column
+    HIPCHK(hipStreamCreate(&s->side.st));
+    HIPCHK(hipEventCreateWithFlags(&s->side.fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&s->side.join, hipEventDisableTiming));
+  }
   x.i0 = i0;
   x.i1 = i1;
   x.j0 = j0;
