@@ -116,7 +116,16 @@ struct Tfsf {
   int tab_h0, tab_n;
   const double* table;
   double P[6];
+  // a dipole with a small box: the incident value of component c at
+  // half-coordinates h in [ptab_h0, ptab_h0 + ptab_n) on every axis, filled
+  // before each correction launch (k_tfsf_points), index ((c * n0 + h0) *
+  // n1 + h1) * n2 + h2 relative to ptab_h0; null: evaluate per point
+  double* ptab;
+  int ptab_h0[3], ptab_n[3];
 };
+// Point tables up to 2^20 values (boxes up to about 24 cells a side): their
+// volume stays near the face bands' own evaluation count.
+constexpr long long kMaxPointTable = 1LL << 20;
 
 template <int STORE> struct Fields;
 template <> struct Fields<0> {

@@ -172,12 +172,13 @@ __device__ __forceinline__ int comp_at(int hx, int hy, int hz) {
 
 // Sum over the 4 stencil neighbours along axis a of component-at-(hp) of the
 // TF/SF corrections (genAlg.py:219-221) weighted by (+1,-27,+27,-1).
-// TAB: plane wave with k along one axis, the incident value is the
-// polarisation factor P[comp] times this half-step's waveform table entry.
+// TAB 1: plane wave with k along one axis, the incident value is the
+// polarisation factor P[comp] times this half-step's waveform table entry;
+// TAB 2: a dipole's point table (k_tfsf_points); TAB 0: evaluated here.
 // The neighbours differ from the point only along a, so no face is crossed
 // unless the point lies inside the box on the other two axes; then a
 // neighbour crosses when its a-coordinate is on the other side of [hlo, hhi].
-template <bool TAB>
+template <int TAB>
 __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int hz, int a, double t,
                                            const double* table) {
   const int hp[3] = {hx, hy, hz};
@@ -198,9 +199,14 @@ __device__ __forceinline__ float tfsf_dcorr(const Tfsf& s, int hx, int hy, int h
     if (qin == pin) continue;
     const bool qsf = s.sf_inside ? qin : !qin;  // the neighbour is in the scattered-field region
     double v;
-    if (TAB) {
+    if constexpr (TAB == 1) {
       const int ha = s.tab_axis == a ? hq : hp[s.tab_axis];
       v = s.P[cq] * table[min(max(ha - s.tab_h0, 0), s.tab_n - 1)];
+    } else if constexpr (TAB == 2) {  // the launch's point table; clamped reads stay in it
+      const int px = min(max((a == 0 ? hq : hx) - s.ptab_h0[0], 0), s.ptab_n[0] - 1);
+      const int py = min(max((a == 1 ? hq : hy) - s.ptab_h0[1], 0), s.ptab_n[1] - 1);
+      const int pz = min(max((a == 2 ? hq : hz) - s.ptab_h0[2], 0), s.ptab_n[2] - 1);
+      v = s.ptab[(((long long)cq * s.ptab_n[0] + px) * s.ptab_n[1] + py) * s.ptab_n[2] + pz];
     } else {
       v = incident(s, cq, a == 0 ? hq : hx, a == 1 ? hq : hy, a == 2 ? hq : hz, t);
     }
@@ -902,7 +908,7 @@ __global__ __launch_bounds__(256) void k_ade(Grid g, Fields<STORE> f, Fields<STO
 // the first band that contains it, and that thread applies the terms of
 // every axis whose band holds the cell, so cells where bands of different
 // axes overlap (box edges and corners) are written by one thread only.
-template <int STORE, int MODE, bool TAB>
+template <int STORE, int MODE, int TAB>
 __global__ __launch_bounds__(256) void k_tfsf(Grid g, Fields<STORE> f, Scales sc, Cpml pm, Materials mt, Ade ade,
                                               Tfsf tf, TfsfBands bands, double tinc) {
   constexpr int HALF = MODE == 1 ? 1 : 0;
@@ -1824,9 +1830,11 @@ static void tfsf_go(const Grid& g, const void* fields, const Scales& sc, const C
   }
   const dim3 grid((unsigned)((n + 255) / 256));
   if (tf.tab_axis >= 0)
-    hipLaunchKernelGGL((k_tfsf<STORE, MODE, true>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, b, tinc);
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE, 1>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, b, tinc);
+  else if (tf.ptab)
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE, 2>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, b, tinc);
   else
-    hipLaunchKernelGGL((k_tfsf<STORE, MODE, false>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, b, tinc);
+    hipLaunchKernelGGL((k_tfsf<STORE, MODE, 0>), grid, dim3(256), 0, st, g, f, sc, pm, mt, ade, tf, b, tinc);
   check_launch("k_tfsf");
 }
 
@@ -1912,9 +1920,29 @@ void launch_tfsf_fold(const Tfsf& tf, const double* tables, float* out, int nste
   check_launch("k_tfsf_fold");
 }
 
+// Incident values of every component on the point table's box at time t:
+// one thread per value, so the FP64 transcendentals of a dipole run across
+// the chip instead of in the band threads one after another.
+__global__ __launch_bounds__(256) void k_tfsf_points(Tfsf tf, double t) {
+  const long long n = 6LL * tf.ptab_n[0] * tf.ptab_n[1] * tf.ptab_n[2];
+  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int p2 = (int)(q % tf.ptab_n[2]);
+  const long long r = q / tf.ptab_n[2];
+  const int p1 = (int)(r % tf.ptab_n[1]);
+  const int p0 = (int)((r / tf.ptab_n[1]) % tf.ptab_n[0]);
+  const int c = (int)(r / tf.ptab_n[1] / tf.ptab_n[0]);
+  tf.ptab[q] = incident(tf, c, tf.ptab_h0[0] + p0, tf.ptab_h0[1] + p1, tf.ptab_h0[2] + p2, t);
+}
+
 void launch_tfsf(int store, int half, const Grid& g, const void* fields, const Scales& sc, const Cpml& pm,
                  const Materials& mt, const Ade& ade, const Tfsf& tf, const TfsfBands& bands, double t_inc,
                  hipStream_t st) {
+  if (tf.ptab && tf.tab_axis < 0) {
+    const long long n = 6LL * tf.ptab_n[0] * tf.ptab_n[1] * tf.ptab_n[2];
+    hipLaunchKernelGGL(k_tfsf_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tf, t_inc);
+    check_launch("k_tfsf_points");
+  }
   if (store == 0 && half == 0) tfsf_go<0, 0>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
   if (store == 0 && half == 1) tfsf_go<0, 1>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);
   if (store == 0 && half == 2) tfsf_go<0, 2>(g, fields, sc, pm, mt, ade, tf, bands, t_inc, st);

@@ -474,6 +474,21 @@ static int tfsf_plan(const dtm_config& c, Tfsf& t, TfsfBands& B) {
 
 static int setup_tfsf(dtm_solver* s) {
   if (int rc = tfsf_plan(s->cfg, s->tf, s->bands)) return rc;
+  s->tf.ptab = nullptr;
+  if (s->tf.kind == DTM_TFSF_DIPOLE) {
+    // every incident value the corrections read lies at half-coordinates
+    // [hlo - 3, hhi + 3] on each axis (a band cell's neighbour across a face
+    // is at most 3 half-cells from it, and the other axes lie in the box)
+    long long tot = 6;
+    for (int a = 0; a < 3; a++) {
+      const long long n = (long long)s->tf.hhi[a] - s->tf.hlo[a] + 7;
+      tot = (n <= 0 || n > kMaxPointTable) ? kMaxPointTable + 1 : std::min(tot * n, kMaxPointTable + 1);
+      s->tf.ptab_h0[a] = s->tf.hlo[a] - 3;
+      s->tf.ptab_n[a] = (int)std::max(0LL, std::min(n, kMaxPointTable));
+    }
+    if (tot <= kMaxPointTable)
+      if (int rc = dalloc(s, &s->tf.ptab, sizeof(double) * (size_t)tot)) return rc;
+  }
   if (s->tf.kind != DTM_TFSF_NONE && s->tf.tab_axis >= 0) {
     if (int rc = dalloc(s, &s->tab_base, sizeof(double) * 2 * kTableSteps * s->tf.tab_n)) return rc;
     // per-plane factors of a wave along x, for the sweeps that fold it in
