@@ -65,6 +65,7 @@ struct Materials {
   const uint8_t* cls;  // nullptr => all cells are class 0
   const ClassEntry* table;
   int nclass;
+  int cx0, cx1;        // the x planes that hold cells of a class other than 0: [cx0, cx1)
   // continuous-eps design box (inverse design)
   int d_on, d_i0, d_j0, d_k0, d_ni, d_nj, d_nk;
   const float* d_inv_eps;

@@ -1138,7 +1138,10 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_e(Grid g, Fields<STORE> 
     S.ex = ldv<P>((const T*)Ex, pb, eq, zi);
     S.ey = ldv<P>((const T*)Ey, pb, eq, zi);
     S.ez = ldv<P>((const T*)Ez, pb, eq, zi);
-    S.cls = ldv<CL>(clsp, mt.cls ? pb : 0, cq, czidx);
+    // planes without a class other than 0 read the class array's zero pad
+    // (one cached line) instead of a byte per cell
+    const int pw = GX ? nbr(p, 0, g.nx, g.per[0]) : p;
+    S.cls = ldv<CL>(clsp, mt.cls ? (pw >= mt.cx0 && pw < mt.cx1 ? pb : -1) : 0, cq, czidx);
   };
   auto ring_direct = [&](int p) {  // Hy / Hz tile of plane p into its slot
     const long long pb = plane_base<GX>(g, p);

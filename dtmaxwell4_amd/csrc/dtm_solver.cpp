@@ -700,12 +700,18 @@ static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]) {
   // per-cell class array exists (cls_of)
   const bool uniform = !m[0] && !m[1] && !m[2] && !s->classes[0].special;
   if (!uniform) s->cls_host.assign(n, 0);
+  int cx0 = c.nx, cx1 = 0;  // x range of cells whose class is not 0
   for (size_t q = 0; q < (uniform ? 0 : n); q++) {
     const uint8_t a = m[0] ? m[0][q] : 0, b = m[1] ? m[1][q] : 0, d = m[2] ? m[2][q] : 0;
     if (a >= c.nmat || b >= c.nmat || d >= c.nmat) return fail(DTM_ERR_ARG, "material id >= nmat");
     const int id = entry_for(a, b, d);
     if (id < 0) return fail(DTM_ERR_LIMIT, "more than 256 distinct (Ex,Ey,Ez) material classes");
     s->cls_host[q] = (uint8_t)id;
+    if (id) {
+      const int i = (int)(q / ((size_t)c.ny * c.nz));
+      cx0 = std::min(cx0, i);
+      cx1 = std::max(cx1, i + 1);
+    }
     if (s->classes[id].special) {
       any_special = true;
       if (s->classes[id].special & 8) any_gyro = true;
@@ -760,6 +766,8 @@ static int build_classes_into(dtm_solver* s, const uint8_t* const m[3]) {
     HIPCHK(hipMemcpy(s->cls_dev, s->cls_host.data(), n, hipMemcpyHostToDevice));
   }
   s->mt.cls = ncls > 1 ? s->cls_dev : nullptr;
+  s->mt.cx0 = ncls > 1 ? cx0 : 0;
+  s->mt.cx1 = ncls > 1 ? cx1 : 0;
   s->mt.table = s->table_dev;
   s->mt.nclass = ncls;
   // dispersive / gyrotropic state over the bounding box of special cells
