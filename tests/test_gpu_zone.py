@@ -4,8 +4,8 @@ components every kernel reads and writes in FP32.
 Checks, each sharp against a missed zone read or write (an FP16 rounding is
 5e-4 of the value, the tolerances below are 1e-6):
 - a zone over the whole grid reproduces FP32 storage with every kernel
-  involved: CPML, the dipole TF/SF box, Drude + Lorentz ADE cells, a soft
-  source, probes and their FP64 DFT;
+  involved: CPML or the split-field layer, the dipole TF/SF box, Drude +
+  Lorentz ADE cells, a soft source, probes and their FP64 DFT;
 - a zone inside the grid, with random initial fields confined to it for the
   steps run (the stencil spreads a field 3 cells a step), reproduces FP32
   storage in every cell, at zone edges that cut the sweep tiles in y and z,
@@ -48,11 +48,14 @@ def purcell_case(storage, steps=1500):
     return w
 
 
-def test_whole_grid_zone_equals_fp32_storage():
+@pytest.mark.parametrize("pml", [abi.PML_CPML, abi.PML_SPLIT])
+def test_whole_grid_zone_equals_fp32_storage(pml):
     w32 = purcell_case(abi.STORE_FP32, steps=400)
+    w32.cfg.pml_kind = pml
     a = configs.apply(w32, _solver(w32.cfg))
     a.step(w32.steps)
     w16 = purcell_case(abi.STORE_FP16, steps=400)
+    w16.cfg.pml_kind = pml
     b = _solver(w16.cfg)
     n = w16.cfg.nx
     b.set_fp32_zone(0, n, 0, n, 0, n)
