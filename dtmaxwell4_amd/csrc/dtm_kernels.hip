@@ -640,8 +640,8 @@ __global__ __launch_bounds__(BZ* ROWS, 4) void k_update_h(Grid g, Fields<STORE> 
     for (int q = 0; q < V; q++) tfw[q] = fold_pack(0, j, k + q, hd);
   }
 
-  // PF prefetch sets, each a plane ahead of the next: three planes in flight
-  // where the registers allow it (the y-CPML variant keeps two).
+  // PF prefetch sets, each a plane ahead of the next: two planes in flight in
+  // the launched variants, which carry the y-CPML (three would spill).
   constexpr int PF = (GY || V == 2) ? 2 : 3;
   PreH<P, T> S[PF];
 #pragma unroll

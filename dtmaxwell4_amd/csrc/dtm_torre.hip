@@ -129,7 +129,8 @@ __device__ __forceinline__ void cpml_d(float po, float* pn, unsigned pq, float b
 // that evaluate the halo H columns keep their own E queue).  LDS then holds
 // only what the y/z stencils need: the E^n tile of the current plane (two
 // buffers by plane parity) and the H tiles of the last four planes, 87 KB
-// instead of 157 KB, and an x derivative costs no LDS traffic.  The plane
+// (LDS rings for the x stencils took 157 KB), and an x derivative costs no
+// LDS traffic.  The plane
 // loop is unrolled by four, so every queue slot and LDS buffer index is a
 // compile-time constant (plane p of a tile lives in slot (p - p0) & 3).
 namespace {
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
     S.cls = ldz(clsp, clsg ? pc : 0, clsg ? ho_e : 0, czidx);
   };
 
-  // CPML state per column / row (as in k_torre)
+  // CPML state per column / row
   const auto& zp = C0->zp;
   const bool pm_on = zp.on;
   const int xlo = zp.plo[0], xhi = zp.phi[0];
