@@ -154,6 +154,69 @@ __host__ __device__ __forceinline__ int exact_idx(const Exact& x, int i, int j, 
   return ((i - x.i0) * (x.j1 - x.j0) + (j - x.j0)) * (x.k1 - x.k0) + (k - x.k0);
 }
 
+// ------------------------------------------- TF/SF folded into the sweeps
+// A plane wave along x (table axis 0): the incident value at every stencil
+// neighbour along y or z of a component is P[cq] T(h_x) with h_x the
+// component's own x half-coordinate, so tfsf_dcorr along y or z separates into
+// a per-plane factor P[cq] T(h_x) [h_x inside the box] and a per-lane integer
+// weight W(h) = sum over the neighbours across the face of wt * (+-1).  Along
+// x it is a per-plane sum X(h_x) = sum wt * (+-1) T(h_x + o) times a per-lane
+// inside flag.  k_tfsf_fold writes the six per-plane factors of each half-step
+// (rd_a folded in); the two-pass sweeps and the fused step add
+// weight * factor  to the six derivatives.
+__host__ __device__ __forceinline__ int fold_w(int h, int lo, int hi, int sf_inside) {
+  const bool pin = h >= lo && h <= hi;
+  int w = 0;
+  const int ho[4] = {-3, -1, 1, 3}, wt[4] = {1, -27, 27, -1};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool qin = h + ho[q] >= lo && h + ho[q] <= hi;
+    if (qin == pin) continue;
+    w += ((sf_inside ? qin : !qin) ? wt[q] : -wt[q]);
+  }
+  return w;  // |w| <= 28
+}
+// Six weights of one cell and half-step, packed: four signed 6-bit fields
+// (bits 0, 6, 12, 18: factors 0, 1, 2, 5) and two flags (bits 24, 25:
+// factors 3, 4).  H half-step (factors: Hx/y, Hx/z, Hy/z, Hy/x, Hz/x, Hz/y):
+//   W_y(2j) zin(2k), W_z(2k) yin(2j), W_z(2k) yin(2j+1), yin(2j+1) zin(2k),
+//   yin(2j) zin(2k+1), W_y(2j) zin(2k+1);
+// E half-step (Ex/y, Ex/z, Ey/z, Ey/x, Ez/x, Ez/y):
+//   W_y(2j+1) zin(2k+1), W_z(2k+1) yin(2j+1), W_z(2k+1) yin(2j), yin(2j) zin(2k+1),
+//   yin(2j+1) zin(2k), W_y(2j+1) zin(2k).
+// hd: hlo_y, hhi_y, hlo_z, hhi_z, sf_inside.
+__host__ __device__ __forceinline__ int fold_pack(int half, int j, int k, const int* hd) {
+  auto yin = [&](int h) { return h >= hd[0] && h <= hd[1] ? 1 : 0; };
+  auto zin = [&](int h) { return h >= hd[2] && h <= hd[3] ? 1 : 0; };
+  auto Wy = [&](int h) { return fold_w(h, hd[0], hd[1], hd[4]); };
+  auto Wz = [&](int h) { return fold_w(h, hd[2], hd[3], hd[4]); };
+  const int y0 = 2 * j, z0 = 2 * k;
+  int f0, f1, f2, f3, f4, f5;
+  if (half == 0) {
+    f0 = Wy(y0) * zin(z0);
+    f1 = Wz(z0) * yin(y0);
+    f2 = Wz(z0) * yin(y0 + 1);
+    f3 = yin(y0 + 1) * zin(z0);
+    f4 = yin(y0) * zin(z0 + 1);
+    f5 = Wy(y0) * zin(z0 + 1);
+  } else {
+    f0 = Wy(y0 + 1) * zin(z0 + 1);
+    f1 = Wz(z0 + 1) * yin(y0 + 1);
+    f2 = Wz(z0 + 1) * yin(y0);
+    f3 = yin(y0) * zin(z0 + 1);
+    f4 = yin(y0 + 1) * zin(z0);
+    f5 = Wy(y0 + 1) * zin(z0);
+  }
+  return (f0 & 63) | ((f1 & 63) << 6) | ((f2 & 63) << 12) | ((f5 & 63) << 18) | (f3 << 24) | (f4 << 25);
+}
+__device__ __forceinline__ float fold_f(int w, int off) {  // signed 6-bit field at bit off
+  return (float)((int)((unsigned)w << (26 - off)) >> 26);
+}
+__device__ __forceinline__ float fold_b(int w, int bit) { return (float)((w >> bit) & 1); }
+// Floats per plane in the factor table; the table of one half-step starts
+// with kFoldStride ints of header (hlo_y, hhi_y, hlo_z, hhi_z, sf_inside).
+constexpr int kFoldStride = 8;
+
 struct Scales {  // FP16 storage: value = stored * inv, stored = value * scale, and the FP32 zone
   float se[3], sh[3], ie[3], ih[3];
   Exact xz;
@@ -310,8 +373,11 @@ void launch_ade(int store, const Grid& g, const void* fields_src, const void* fi
                 const Materials& mt, const Ade& ade, hipStream_t st);
 // Fused time step (dtm_torre.hip): old fields/psi -> new fields/psi.
 // `cold` is a device copy of the step's TorreCold (make_torre_cold).
+// fold non-null: a plane-wave TF/SF source along x added inside the step,
+// from the step's factor tables (the H half-step's, the E half-step's
+// following it at +(nx + 1) * kFoldStride).
 void launch_torre(int store, const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
-                  const Region* regs, int nreg, hipStream_t st);
+                  const Region* regs, int nreg, hipStream_t st, const float* fold = nullptr);
 TorreCold make_torre_cold(const void* fields_old, const void* fields_new, int store, const Cpml& pm_old,
                           const Cpml& pm_new, const Materials& mt, const Ade& ade);
 void launch_soft(int store, const Grid& g, const void* fields, const Scales& sc, const Soft& so,

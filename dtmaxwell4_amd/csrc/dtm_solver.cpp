@@ -126,7 +126,7 @@ struct dtm_solver {
 };
 
 // floats of one half-step's fold factor table: header + nx planes, 8 each
-static long long fold_floats(const dtm_solver* s) { return ((long long)s->cfg.nx + 1) * 8; }
+static long long fold_floats(const dtm_solver* s) { return ((long long)s->cfg.nx + 1) * kFoldStride; }
 
 template <typename T>
 static int dalloc(dtm_solver* s, T** p, size_t bytes) {
@@ -491,8 +491,8 @@ static int setup_tfsf(dtm_solver* s) {
   }
   if (s->tf.kind != DTM_TFSF_NONE && s->tf.tab_axis >= 0) {
     if (int rc = dalloc(s, &s->tab_base, sizeof(double) * 2 * kTableSteps * s->tf.tab_n)) return rc;
-    // per-plane factors of a wave along x, for the sweeps that fold it in
-    if (s->tf.tab_axis == 0 && !s->torre)
+    // per-plane factors of a wave along x, for the sweeps (and the fused step) that fold it in
+    if (s->tf.tab_axis == 0)
       if (int rc = dalloc(s, &s->fold_base, sizeof(float) * 2 * kTableSteps * fold_floats(s))) return rc;
   }
   return 0;
@@ -1150,22 +1150,28 @@ static int one_step(dtm_solver* s) {
   if (rec && s->fs.nface > 0) launch_flux_snap(s->store, s->g, F, s->sc, s->fs, s->stream);
   const double th = (double)n * c.dt, te = ((double)n + 0.5) * c.dt;
   if (s->torre) {
-    // TF/SF on H^{n-1/2} before the fused step (exact: the step is linear)
-    if (s->tf.active) {
+    // A plane wave along x is folded into the fused step (both half-steps'
+    // factor tables); other sources apply k_tfsf's H correction to
+    // H^{n-1/2} before the step (exact: the step is linear) and its E
+    // correction after it.
+    const bool fold = s->tf.active && s->fold_base && !s->no_fold;
+    Tfsf tfh{};
+    if (s->tf.active) tfh = tfsf_for(s, n, 0);
+    if (s->tf.active && !fold) {
       ev_begin(s, 4);
-      launch_tfsf(s->store, 2, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 0), s->bands, th, s->stream);
+      launch_tfsf(s->store, 2, s->g, F, s->sc, s->pm, s->mt, s->ade, tfh, s->bands, th, s->stream);
       ev_end(s);
     }
     const TorreCold* cold = torre_cold(s);
     if (!cold) return DTM_ERR_DEVICE;
     ev_begin(s, 0);
-    launch_torre(s->store, s->g, s->sc, s->mt, cold, &s->whole, 1, s->stream);
+    launch_torre(s->store, s->g, s->sc, s->mt, cold, &s->whole, 1, s->stream, fold ? fold_for(s, n, 0) : nullptr);
     ev_end(s);
     ev_begin(s, 4);
     launch_ade(s->store, s->g, F, fields_next_ptr(s), s->sc, s->mt, s->ade, s->stream);
     swap_buffers(s);
     F = fields_ptr(s);
-    if (s->tf.active)
+    if (s->tf.active && !fold)
       launch_tfsf(s->store, 1, s->g, F, s->sc, s->pm, s->mt, s->ade, tfsf_for(s, n, 1), s->bands, te, s->stream);
   } else {
     // A plane wave along x is folded into the sweeps (k_tfsf's terms added to

@@ -41,6 +41,7 @@ constexpr int ER = TR + 3;        // E tile rows (19): j0-3 .. j0+15
 constexpr int EC = TZ + 6;        // E tile cols (70): k0-3 .. k0+66
 constexpr int ETILE = ER * EC;    // 1330
 constexpr int NXCD = 8;
+constexpr int kFoldPlanes = 264;  // planes of a block's fold table: segments of at most kFoldPlanes - 3
 
 template <typename T> __device__ __forceinline__ float cv(T v, float inv);
 template <> __device__ __forceinline__ float cv<float>(float v, float) { return v; }
@@ -168,14 +169,24 @@ template <typename T> struct PreT2 {
 };
 }  // namespace
 
-template <int STORE, int GEN>
+// TF: a plane-wave TF/SF source along x folded in, as in the two-pass sweeps:
+// every H evaluation (own and halo columns) adds its weights times the H
+// half-step's factors of plane p, every E update those of the E half-step at
+// plane m.  fold: the H half-step's factor table, the E half-step's after it.
+template <int STORE, int GEN, bool TF>
 __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint8_t* clsg, const TorreCold* cold,
-                                                   TorreSet ts) {
+                                                   TorreSet ts, const float* fold) {
   using T = typename std::conditional<STORE == 0, float, __half>::type;
   constexpr bool GX = GEN & 1, GY = GEN & 2;
   __shared__ ClassEntry tbl[kMaxClasses];
   __shared__ float sE[2][3][ETILE];   // E^n tile of plane p, buffer (p - p0) & 1
   __shared__ float sH[4][3][TR][HC];  // H^{n+1/2} tiles, plane q in slot (q - p0) & 3
+  // TF/SF fold factors of the block's planes p0 + l: the H half-step's at
+  // [l][0-5], the E half-step's at [l][8-13]
+  __shared__ __align__(16) float sF[TF ? kFoldPlanes : 1][16];
+  // and each thread's packed weights (kept in LDS, not in registers held
+  // across the plane loop): [0] own H column, [1] halo H column (wave 0) or own E cell
+  __shared__ int sW[TF ? 2 : 1][TR][TZ];
   const int tx = threadIdx.x;
   const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one row per wavefront
   const int tid = ty * TZ + tx;
@@ -276,6 +287,24 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
   const int di0 = mc.d_i0, di1 = mc.d_i0 + mc.d_ni;
   const float rd0 = g.rd[0], rd1 = g.rd[1], rd2 = g.rd[2], dt = g.dt;
   const int p0 = t.i0 - 1, pend = t.i1 + 1;  // H planes p0 .. pend
+  // TF/SF fold: packed weights of the own H column, and of the halo H column
+  // (wave 0, which owns no E) or the own E cell (zero away from the y and z faces)
+  const float* foldE = fold + (long long)(g.nx + 1) * kFoldStride;
+  if constexpr (TF) {
+    const int* hd = reinterpret_cast<const int*>(fold);
+    int tw2 = 0;
+    if (ty == 0) tw2 = hv_h ? fold_pack(0, jmh, kmh, hd) : 0;
+    else if (hv_m) tw2 = fold_pack(1, jm, kmm, hd);
+    sW[0][ty][tx] = hv_m ? fold_pack(0, jm, kmm, hd) : 0;
+    sW[TF ? 1 : 0][ty][tx] = tw2;
+    // the factor tables of planes p0 .. pend (the launch keeps them within
+    // kFoldPlanes); the first barrier of the plane loop orders these writes
+    for (int q = tid; q < (pend - p0 + 1) * 16; q += TZ * TR) {
+      const int l = q >> 4, c = q & 15;
+      const int ii = min(max(GX ? nbr(p0 + l, g.nx, g.per[0]) : p0 + l, 0), g.nx - 1);
+      sF[l][c] = (c & 7) < 6 ? (c < 8 ? fold : foldE)[kFoldStride * (1 + ii) + (c & 7)] : 0.f;
+    }
+  }
 
   // queues: qE[c][s] E^n of the own column, qG[c][s] Ey / Ez of the halo
   // column, qH[c][s] Hy / Hz of the own column; plane q in slot (q - p0) & 3
@@ -303,7 +332,8 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
   // H from the E tile (y/z) and the x queue (planes p-2 .. p+1 in ex[0..3])
   auto h_new = [&](const ColdK* C, const float* tE, int e, const float* exy, const float* exz, int ip, int jr,
                    int km, int zs, float bz, float cz, int ys, float by, float cy, bool own, float hx0, float hy0,
-                   float hz0, float pz0, float pz1, float py0, float py1, float& hx, float& hy, float& hz) {
+                   float hz0, float pz0, float pz1, float py0, float py1, int tw, const float* fu, float& hx,
+                   float& hy, float& hz) {
     const float* tEx = tE;
     const float* tEy = tE + ETILE;
     const float* tEz = tE + 2 * ETILE;
@@ -314,6 +344,16 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
     float dzEx = rd2 * d24(tEx[e - 2], tEx[e - 1], ex0, tEx[e + 1]);
     float dxEz = rd0 * d24(exz[0], exz[1], exz[2], exz[3]);
     float dxEy = rd0 * d24(exy[0], exy[1], exy[2], exy[3]);
+    if constexpr (TF) {
+      if (tw) {
+        dyEz += fold_f(tw, 0) * fu[0];   // Hx, y faces
+        dzEy += fold_f(tw, 6) * fu[1];   // Hx, z faces
+        dzEx += fold_f(tw, 12) * fu[2];  // Hy, z faces
+        dxEz += fold_b(tw, 24) * fu[3];  // Hy, x faces
+        dxEy += fold_b(tw, 25) * fu[4];  // Hz, x faces
+        dyEx += fold_f(tw, 18) * fu[5];  // Hz, y faces
+      }
+    }
     if (zs >= 0) {
       const unsigned pq = psi_index(g, Pz, 2, zs, ip, jr, km);
       cpml_d(pz0, C->zp.hN[0], pq, bz, cz, dzEy, own);
@@ -367,6 +407,9 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
 #pragma unroll
       for (int c = 0; c < 3; c++) tE[c * ETILE + hpos] = hE[c];
     }
+    // the fold factors of H plane p and E plane m (read only by lanes next to a face)
+    const float* fuH = sF[TF ? p - p0 : 0];
+    const float* fuE = sF[TF ? max(m - p0, 0) : 0] + 8;
     // 3) psi and design operands of this plane, then the next iteration's loads
     const long long pb = plane(p);
     const int ip = GX ? nbr(p, g.nx, g.per[0]) : p;
@@ -420,7 +463,7 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
       const float exy[4] = {qE[1][s2], qE[1][s3], qE[1][s0], qE[1][s1]};
       const float exz[4] = {qE[2][s2], qE[2][s3], qE[2][s0], qE[2][s1]};
       h_new(C, tE, emain, exy, exz, ip, jm, kmm, zsm, bzm, czm, ysl, byh, cyh, hown, oH[0], oH[1], oH[2], pzm0,
-            pzm1, pym0, pym1, hx, hy, hz);
+            pzm1, pym0, pym1, TF ? sW[0][ty][tx] : 0, fuH, hx, hy, hz);
     }
     sH[s0][0][ty][tx + 1] = hx;
     sH[s0][1][ty][tx + 1] = hy;
@@ -439,7 +482,7 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
         const float gey[4] = {qG[0][s2], qG[0][s3], qG[0][s0], qG[0][s1]};
         const float gez[4] = {qG[1][s2], qG[1][s3], qG[1][s0], qG[1][s1]};
         h_new(C, tE, eh, gey, gez, ip, jmh, kmh, zsh, bzh, czh, yslh, byhh, cyhh, false, oG[0], oG[1], oG[2], pzh0,
-              pzh1, pyh0, pyh1, gx, gy, gz);
+              pzh1, pyh0, pyh1, TF ? sW[TF ? 1 : 0][ty][tx] : 0, fuH, gx, gy, gz);
       }
       if (hal) {
         sH[s0][0][hr][chh] = gx;
@@ -460,6 +503,17 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
       float dzHx = rd2 * d24(h0[0][ty][c - 1], hx0, h0[0][ty][c + 1], h0[0][ty][c + 2]);
       float dxHz = rd0 * d24(qH[1][s1], qH[1][s2], qH[1][s3], qH[1][s0]);
       float dxHy = rd0 * d24(qH[0][s1], qH[0][s2], qH[0][s3], qH[0][s0]);
+      if constexpr (TF) {
+        const int tw2 = sW[TF ? 1 : 0][ty][tx];
+        if (tw2) {
+          dyHz += fold_f(tw2, 0) * fuE[0];   // Ex, y faces
+          dzHy += fold_f(tw2, 6) * fuE[1];   // Ex, z faces
+          dzHx += fold_f(tw2, 12) * fuE[2];  // Ey, z faces
+          dxHz += fold_b(tw2, 24) * fuE[3];  // Ey, x faces
+          dxHy += fold_b(tw2, 25) * fuE[4];  // Ez, x faces
+          dyHx += fold_f(tw2, 18) * fuE[5];  // Ez, y faces
+        }
+      }
       if (zsm >= 0) {
         const unsigned pq = psi_index(g, Pz, 2, zsm, im, jm, kmm);
         cpml_d(pze0, C->zp.eN[0], pq, bze, cze, dzHy, true);
@@ -540,8 +594,9 @@ TorreSet make_torre_set(const Region* regs, int nreg, int seg) {
 
 // Segment length: a block computes seg + 3 H planes (one halo plane below,
 // two above) after a warm-up of about kWarm planes, and one launch runs in
-// ceil(blocks / slots) rounds; minimise rounds * (seg + 3 + kWarm).
-static int torre_seg(const Region* regs, int nreg, int slots) {
+// ceil(blocks / slots) rounds; minimise rounds * (seg + 3 + kWarm) over
+// segments of at most maxseg planes.
+static int torre_seg(const Region* regs, int nreg, int slots, int maxseg) {
   constexpr int kWarm = 6;
   int maxlen = 1;
   for (int t = 0; t < nreg; t++) {
@@ -554,6 +609,7 @@ static int torre_seg(const Region* regs, int nreg, int slots) {
   for (int nseg = 1; nseg <= maxlen; nseg++) {
     const int seg = (maxlen + nseg - 1) / nseg;
     if (nseg > 1 && seg == (maxlen + nseg - 2) / (nseg - 1)) continue;
+    if (seg > maxseg) continue;
     long long blocks = 0;
     for (int t = 0; t < nreg; t++) {
       const Region& r = regs[t];
@@ -586,14 +642,17 @@ static int torre_slots(K kernel) {
 
 template <int STORE>
 static void torre_go(const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
-                     const Region* regs, int nreg, hipStream_t st) {
+                     const Region* regs, int nreg, hipStream_t st, const float* fold) {
   const dim3 blk(TZ, TR, 1);
-  static const int slots = torre_slots(k_torre2<STORE, 3>);
-  TorreSet te = make_torre_set(regs, nreg, torre_seg(regs, nreg, slots));
-  if (te.total > 0) {
-    hipLaunchKernelGGL((k_torre2<STORE, 3>), dim3(te.per * NXCD), blk, 0, st, g, sc, mt.cls, cold, te);
-    check_launch("k_torre2");
-  }
+  static const int slots = torre_slots(k_torre2<STORE, 3, false>);
+  // a folded source keeps each block's factor table within kFoldPlanes
+  TorreSet te = make_torre_set(regs, nreg, torre_seg(regs, nreg, slots, fold ? kFoldPlanes - 3 : 1 << 30));
+  if (te.total <= 0) return;
+  if (fold)
+    hipLaunchKernelGGL((k_torre2<STORE, 3, true>), dim3(te.per * NXCD), blk, 0, st, g, sc, mt.cls, cold, te, fold);
+  else
+    hipLaunchKernelGGL((k_torre2<STORE, 3, false>), dim3(te.per * NXCD), blk, 0, st, g, sc, mt.cls, cold, te, fold);
+  check_launch("k_torre2");
 }
 
 template <int STORE>
@@ -637,9 +696,9 @@ TorreCold make_torre_cold(const void* fields_old, const void* fields_new, int st
 }
 
 void launch_torre(int store, const Grid& g, const Scales& sc, const Materials& mt, const TorreCold* cold,
-                  const Region* regs, int nreg, hipStream_t st) {
-  if (store == 0) torre_go<0>(g, sc, mt, cold, regs, nreg, st);
-  else torre_go<1>(g, sc, mt, cold, regs, nreg, st);
+                  const Region* regs, int nreg, hipStream_t st, const float* fold) {
+  if (store == 0) torre_go<0>(g, sc, mt, cold, regs, nreg, st, fold);
+  else torre_go<1>(g, sc, mt, cold, regs, nreg, st, fold);
 }
 
 }  // namespace dtm

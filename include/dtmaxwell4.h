@@ -287,13 +287,13 @@ int dtm_energy_box(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1
  * under stream capture, tight serving loops). */
 int dtm_energy_box_noalloc(dtm_solver* s, int i0, int i1, int j0, int j1, int k0, int k1, double* out);
 
-/* Where the two-pass schedule applies a plane-wave TF/SF source along x:
- * on (the default) inside the H and E sweeps, added to the derivatives the
- * face terms of genAlg.py:219-221 belong to, from per-plane factors; off: in
- * a separate correction kernel after each sweep.  Both give the same fields
- * up to rounding (the updates are linear in those terms).  Other plane
- * waves, dipoles, the fused step and a plane wave with the FP32 zone always
- * use the separate kernel. */
+/* Where a plane-wave TF/SF source along x is applied: on (the default)
+ * inside the H and E sweeps, or inside the fused step, added to the
+ * derivatives the face terms of genAlg.py:219-221 belong to, from per-plane
+ * factors; off: in a separate correction kernel after each sweep (before and
+ * after the fused step).  Both give the same fields up to rounding (the
+ * updates are linear in those terms).  Other plane waves, dipoles and a
+ * plane wave with the FP32 zone always use the separate kernel. */
 int dtm_set_tfsf_fold(dtm_solver* s, int on);
 
 /* ------------------------------------------------------------ FP32 zone */
