@@ -59,3 +59,27 @@ def test_headline_fp16_storage_tracks_fp32():
     err = rel_l2(b, a)
     print(f"fp16 vs fp32 rel-L2 {err:.3e}")
     assert err < 2e-2
+
+
+def test_fused_fold_with_capped_segments():
+    """416 x 512 columns make 256 column tiles, as many as the blocks an
+    MI355X holds at once, so the fused step's cost model would march each
+    tile over all 300 x planes in one round; with the plane wave folded in,
+    its per-block factor table caps segments at 261 planes, and every column
+    is split in two (150 planes each).  The fused step (folded) must still
+    match the two-pass step (folded)."""
+    from dtmaxwell4_amd.solver import Solver
+    out = []
+    for sweep in (abi.SWEEP_TWOPASS, abi.SWEEP_FUSED):
+        w = configs.plane_wave_sphere(nx=300, ny=416, nz=512, radius=40, steps=300, periods_w=1.0,
+                                      periods_t0=2.0)
+        w.cfg.sweep = sweep
+        s = configs.apply(w, Solver(w.cfg))
+        s.step(300)
+        out.append(s.fields())
+        s.close()
+    assert all(np.isfinite(x).all() for x in out[1])
+    assert np.abs(out[0][abi.EZ]).max() > 0.1
+    err = rel_l2(out[1], out[0])
+    print(f"fused vs two-pass rel-L2 {err:.3e}")
+    assert err <= 1e-5
