@@ -261,6 +261,35 @@ def test_tfsf_folded_into_the_sweeps_equals_the_correction_kernel(storage, pml, 
     assert err <= (1e-6 if storage == abi.STORE_FP32 else 1e-2)
 
 
+def test_tfsf_fold_on_a_dispersive_sphere(sweep):
+    """A plane wave along x on a Drude + Lorentz sphere that crosses the TF/SF
+    box's y faces, so face-band cells are dispersive: the folded terms reach
+    them through the curl the E update stores for k_ade, and the correction
+    kernel's dispersive branch (C3 on E, kJ2 on J) handles them on the other
+    path.  Both against the FP64 oracle (measured 4.4e-6 on both schedules,
+    either path), and folded against the kernel path (the same terms, so
+    rounding-level differences: measured 1.5e-6; a Drude damping of 0.15
+    instead of 1.5 raises both to 1.1e-5 and 2e-6, FP32 rounding that the
+    weakly damped pole accumulates, the same with or without band cells in
+    the sphere)."""
+    w = configs.plane_wave_sphere(nx=56, ny=40, nz=48, npml=8, inset=4, radius=10, steps=300, periods_w=1.0,
+                                  periods_t0=3.0)
+    m = w.cfg.mat[1]
+    m.eps_inf = 2.0
+    m.npoles = 2
+    wd = 2 * np.pi * 1.5
+    m.pole[0].setDrude(1.0, wd, 1.5, 0.0)
+    m.pole[1].set(0.5, wd, 0.1 * wd, 0.0)
+    g, o = run_pair(w)
+    check_fields(g, o)
+    b = configs.apply(w, solver(w.cfg))
+    b.set_tfsf_fold(False)
+    b.step(w.steps)
+    err = rel_l2(g.fields(), b.fields())
+    print(f"folded vs kernel rel-L2 {err:.3e}")
+    assert err <= 5e-6
+
+
 @pytest.mark.parametrize("inset", [4, 0])  # 0: the TF/SF bands reach into the layer
 def test_split_field_layer_plane_wave(inset, sweep):
     """The reference's split-field absorbing layer (genAlg.py:117-148)."""
