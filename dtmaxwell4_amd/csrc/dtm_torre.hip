@@ -289,8 +289,8 @@ __global__ __launch_bounds__(TZ* TR) void k_torre2(Grid g, Scales sc, const uint
   const int p0 = t.i0 - 1, pend = t.i1 + 1;  // H planes p0 .. pend
   // TF/SF fold: packed weights of the own H column, and of the halo H column
   // (wave 0, which owns no E) or the own E cell (zero away from the y and z faces)
-  const float* foldE = fold + (long long)(g.nx + 1) * kFoldStride;
   if constexpr (TF) {
+    const float* foldE = fold + (long long)(g.nx + 1) * kFoldStride;
     const int* hd = reinterpret_cast<const int*>(fold);
     int tw2 = 0;
     if (ty == 0) tw2 = hv_h ? fold_pack(0, jmh, kmh, hd) : 0;
