@@ -290,6 +290,23 @@ def test_tfsf_fold_on_a_dispersive_sphere(sweep):
     assert err <= 5e-6
 
 
+def test_tfsf_fold_periodic_y_z(sweep):
+    """The folded plane wave in a grid periodic along y and z (CPML along x
+    only), with the TF/SF box's y faces inside the grid and no z faces: the
+    fold weights come from the wrapped cell coordinates, and the fused step's
+    halo columns wrap as well (nz = 32, half a tile).  Against the FP64 oracle."""
+    nx, ny, nz, d = 64, 24, 32, 0.05
+    w = configs.plane_wave_sphere(nx=nx, ny=ny, nz=nz, npml=8, inset=4, radius=4, steps=300, periods_w=1.0,
+                                  periods_t0=3.0)
+    cfg = w.cfg
+    cfg.npml[:] = [8, 8, 0, 0, 0, 0]
+    cfg.bc[1] = cfg.bc[2] = abi.BC_PERIODIC
+    cfg.tfsf.box[:] = [12 * d, (nx - 12) * d, 6 * d, (ny - 6) * d, -1e9, 1e9]
+    g, o = run_pair(w)
+    assert np.abs(o.fields()[abi.EZ]).max() > 0.1
+    check_fields(g, o)
+
+
 @pytest.mark.parametrize("inset", [4, 0])  # 0: the TF/SF bands reach into the layer
 def test_split_field_layer_plane_wave(inset, sweep):
     """The reference's split-field absorbing layer (genAlg.py:117-148)."""
