@@ -33,7 +33,7 @@ abi.TFSFsrc.set = _set
 abi.TFSFsrc.check = _check
 TFSFsrc = abi.TFSFsrc
 
-_INT = ("Tsteps", "GridNx", "GridNy", "GridNz", "Npmlx", "Npmly", "Npmlz", "Ntime", "NyBloch", "Nzw", "Np")
+_INT = ("Tsteps", "DropData", "GridNx", "GridNy", "GridNz", "Npmlx", "Npmly", "Npmlz", "Ntime", "NyBloch", "Nzw", "Np")
 _FLOAT = ("ds", "da", "dv", "dt")
 _CONST = ("NDT",)
 

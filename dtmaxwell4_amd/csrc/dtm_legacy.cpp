@@ -30,6 +30,7 @@
 extern "C" {
 int Tsteps = 22 * 10;  // DTmxw.cu:730 (Ntime*10)
 TFSFsrc shotpoint;
+int DropData = 0;  // py_consts.h:19: DROP_DATA is off in the reference's default build
 int GridNx = 600, GridNy = 639, GridNz = 300;  // true x, y, z extents of the reference default
 float ds = 0.005f, da = 0.005f, dv = 0.005f, dt = 0.002f;  // py_consts.h:71
 const int NDT = 3;
@@ -207,6 +208,17 @@ int calcStep(void) {
          "%dx%dx%d=%llu cells | isTFSF=%d\n",
          g_run.iStep, 0, 0, ms, 0.0, 1.e-9 * yee_cells / (ms * 1.e-3), GridNx, GridNy, GridNz,
          (unsigned long long)GridNx * GridNy * GridNz, isTFSF);  // diamond.cu:483-485
+  // SeismoDrops::drop after the step (diamond.cu:469-470, drop.cu:75-107)
+  for (int c = 0; c < 6; c++) {
+    if (!((DropData >> c) & 1)) continue;
+    static const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+    char fname[1024];
+    snprintf(fname, sizeof(fname), "%s/rag%s_%05d.arr", g_run.drop_dir.c_str(), names[c], g_run.iStep);
+    if (dtm_write_arr(g_run.s, c, fname) < 0) {
+      fprintf(stderr, "calcStep: %s\n", dtm_last_error());
+      return -1;
+    }
+  }
   g_run.iStep++;
   fflush(stdout);
   return 0;
@@ -259,6 +271,18 @@ int _main(int argc, char** argv) {
   }
   printf("No GL\n");
   mkdir(drop_dir.c_str(), 0700);
+  if (DropData) {  // SeismoDrops::init (drop.cu:26-38): remove the snapshots of an earlier run
+    static const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+    for (int it = 0;; it++) {
+      int gone = 0;
+      for (int c = 0; c < 6; c++) {
+        char fname[1024];
+        snprintf(fname, sizeof(fname), "%s/rag%s_%05d.arr", drop_dir.c_str(), names[c], it);
+        if (remove(fname) != 0) gone++;
+      }
+      if (gone == 6) break;
+    }
+  }
   dtm_config c;
   dtm_config_defaults(&c, GridNx, GridNy, GridNz);
   c.dx = ds;

@@ -369,6 +369,14 @@ long long dtm_group_cells(const dtm_solver* s, int g);
 /* Globals of the reference (DTmxw.cu:729-730, signal.hpp:12, py_consts.h). */
 extern int Tsteps;
 extern TFSFsrc shotpoint;
+/* Field snapshots after every calcStep, the reference's compile-time
+ * DROP_DATA build (py_consts.h:19, diamond.cu:469-470, drop.cu:75-107) as a
+ * run-time switch: bit c (0..5: Ex..Hz) writes <drop_dir>/rag<comp>_<n>.arr
+ * (n: the calcStep count, %05d) in the aivlib layout of dtm_write_arr.  0 (the
+ * default) is the reference's default build; 4 (Ez) is its DROP_DATA build
+ * (drop.cu:43-44).  _main removes such files left in drop_dir, as the
+ * reference's SeismoDrops::init does. */
+extern int DropData;
 extern int GridNx, GridNy, GridNz;     /* true x, y, z extents (cells) */
 extern float ds, da, dv, dt;           /* py_consts.h:71 */
 extern const int NDT;                  /* py_consts.h:1 */

@@ -90,7 +90,7 @@ def test_main_and_calcstep_write_reference_outputs(tmp_path, capfd):
     from oracle.oracle import Oracle
     assert abi.load_library().dtm_device_count() > 0, "no HIP device visible"
     cv = DTmxw.cvar
-    saved = {k: getattr(cv, k) for k in ("GridNx", "GridNy", "GridNz", "Tsteps")}
+    saved = {k: getattr(cv, k) for k in ("GridNx", "GridNy", "GridNz", "Tsteps", "DropData")}
     try:
         cv.GridNx, cv.GridNy, cv.GridNz = NX, NY, NZ
         dx, dy, dz, dt = cv.ds, cv.da, cv.dv, cv.dt
@@ -110,6 +110,10 @@ def test_main_and_calcstep_write_reference_outputs(tmp_path, capfd):
         # a sensor near the dipole, where the field has arrived after 3 Ntime steps
         extra = (int(round(center[0] / dx)) + 7, int(round(center[1] / dy)), int(round(center[2] / dz)) + 2)
         drop = str(tmp_path / "drop")
+        # the DROP_DATA snapshots of Ez and Hx; a stale file of an earlier run goes
+        cv.DropData = (1 << abi.EZ) | (1 << abi.HX)
+        os.makedirs(drop)
+        open(os.path.join(drop, "ragEy_00000.arr"), "wb").close()
         rc = DTmxw._main(["DTmxw", "--test", "--drop_dir", drop, "--sensor", "%d %d %d" % extra])
         assert rc == 0
         assert DTmxw.calcStep() == 0      # a driver may keep stepping after _main
@@ -160,6 +164,7 @@ def test_main_and_calcstep_write_reference_outputs(tmp_path, capfd):
         o.step(ntime)
         ref.append([o.probe_now(1, names.index(f), x, y, z) for (f, x, y, z) in keys])
     ref = np.array(ref)
+    of = o.fields()
     o.close()
     got = np.stack([series[k] for k in keys], 1)
     scale = np.abs(ref).max()
@@ -170,3 +175,18 @@ def test_main_and_calcstep_write_reference_outputs(tmp_path, capfd):
     print("sensor files vs FP64 oracle: max |diff| / max |value|", err)
     # %g keeps 6 significant digits
     assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 2e-6 * scale)
+
+    # DROP_DATA snapshots (drop.cu:75-107): one per calcStep and dropped
+    # component, the aivlib header then x-major FP32 values
+    assert not os.path.exists(os.path.join(drop, "ragEy_00000.arr"))
+    for c, name in ((abi.EZ, "Ez"), (abi.HX, "Hx")):
+        for it in range(4):
+            path = os.path.join(drop, "rag%s_%05d.arr" % (name, it))
+            raw = np.fromfile(path, dtype=np.int32, count=9)
+            assert raw.tolist() == [12, 0, 0, 0, 3, 4, NZ, NY, NX], raw
+            assert os.path.getsize(path) == 36 + 4 * NX * NY * NZ
+        last = np.fromfile(path, dtype=np.float32, offset=36).reshape(NX, NY, NZ)
+        e = np.linalg.norm(last - of[c]) / np.linalg.norm(of[c])
+        print("rag%s_00003.arr vs FP64 oracle rel-L2" % name, e)
+        assert e <= 1e-5
+    assert not os.path.exists(os.path.join(drop, "ragEz_00004.arr"))
