@@ -74,6 +74,7 @@ def test_legacy_globals_have_reference_defaults():
     assert abs(C.c_float.in_dll(lib, "dt").value - 0.002) < 1e-9  # py_consts.h:71
     assert abs(C.c_float.in_dll(lib, "ds").value - 0.005) < 1e-9
     assert C.c_int.in_dll(lib, "Tsteps").value == 220        # DTmxw.cu:730 (Ntime*10)
+    assert C.c_int.in_dll(lib, "DropData").value == 0        # py_consts.h:19: DROP_DATA off by default
 
 
 def test_tfsfsrc_set_matches_reference_formulas():
