@@ -25,7 +25,7 @@ from dtmaxwell4_amd import abi
 pytestmark = pytest.mark.gpu
 
 NX, NY, NZ = 150, 150, 100
-RATE = re.compile(r"^Step (\d+) /node 0/ subnode 0/: Time +[0-9.]+ ms \|drop +[0-9.]+% \|\|rate +[0-9.]+ "
+RATE = re.compile(r"^Step (\d+) /node 0/ subnode 0/: Time +[0-9.]+ ms \|drop +([0-9.]+)% \|\|rate +[0-9.]+ "
                   r"GYee_cells/sec \|total grid (\d+)x(\d+)x(\d+)=(\d+) cells \| isTFSF=([01])$", re.M)
 
 
@@ -126,8 +126,9 @@ def test_main_and_calcstep_write_reference_outputs(tmp_path, capfd):
     lines = RATE.findall(out)
     assert [int(m[0]) for m in lines] == [0, 1, 2, 3], out[-2000:]
     for m in lines:
-        assert tuple(int(v) for v in m[1:5]) == (NX, NY, NZ, NX * NY * NZ)
-        assert m[5] == "1"
+        assert tuple(int(v) for v in m[2:6]) == (NX, NY, NZ, NX * NY * NZ)
+        assert m[6] == "1"
+        assert 0.0 < float(m[1]) < 100.0  # the share of the snapshot writes (DropData on)
     # default sensor point of init_index and the --sensor point
     X0 = int(round((SS.BoxMs + SS.BoxPs) * 0.5 / dx))
     Y0 = int(round((SS.BoxMa + SS.BoxPa) * 0.5 / dy))
